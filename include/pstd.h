@@ -1,0 +1,198 @@
+/*
+ * pstd.h -- C ABI of the MI355X-native PSTD_QScat time-stepping core.
+ *
+ * Everything here is plain C: POD structs, pointers and sizes, int error
+ * codes.  No C++ exception and no torch type crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces
+ * (UnitedTabby/PSTD-ScalarWaveQuantumScattering, file:line).
+ *
+ * Error codes reuse the reference's errno-style numbers
+ * (qscat.cpp:75-79):
+ *    0  success
+ *    2  subdomain grid / decomposition inconsistent (PSTD_QScat.cpp:244-249)
+ *    3  file I/O error                              (qscat.cpp:225-226, PSTD_QScat.cpp:2429)
+ *    4  incomplete parameter file                   (qscat.cpp:295-296)
+ *   12  ENOMEM, device or host allocation failed    (PSTD_QScat.cpp:2557-2564)
+ *   22  EINVAL, invalid argument
+ *   19  ENODEV, no usable MI355X (gfx950) device / HIP runtime error
+ * pstd_last_error() returns the message of the last failure on the calling
+ * thread.
+ */
+#ifndef PSTD_H
+#define PSTD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSTD_OK        0
+#define PSTD_ETOPO     2
+#define PSTD_EIO       3
+#define PSTD_EINPUT    4
+#define PSTD_ENOMEM   12
+#define PSTD_EINVAL   22
+#define PSTD_ENODEV   19
+
+/* Number of windowed halo planes; PSTD_QScat.h:56 (NWEIGHT). */
+#define PSTD_NWEIGHT 8
+
+/* Storage / arithmetic precision of psi.  8 = FP64 everywhere (the
+ * reference's dqscat); 4 = FP32 psi, operators and updates with the incident
+ * phase, TF/SF source and surface phase accumulation kept in FP64. */
+#define PSTD_FP64 8
+#define PSTD_FP32 4
+
+/* Derivative selector for surface output; PSTD_QScat.h:53 (DerivativeType). */
+#define PSTD_DERIV_PS 0
+#define PSTD_DERIV_FD 1
+
+/* The ten fields of the reference input file, in file order
+ * (qscat.cpp:228-293), plus the precision switch that the reference selects
+ * at compile time (run_environment.h:36-78, makefile:7-15). */
+typedef struct pstd_params {
+    int    nprocs[3];      /* Px Py Pz: subdomain grid                     */
+    double dtau;           /* time step, radians of omega*t                */
+    double d[3];           /* dx dy dz, radians of k*x                     */
+    long   L[3];           /* local FFT length per axis                    */
+    long   n_abc;          /* absorbing layer thickness                    */
+    long   n_sca;          /* scattered-field zone thickness               */
+    long   n_ksi;          /* TF/SF transition layer thickness             */
+    long   n_halo;         /* overlap (halo) thickness h                   */
+    double theta, phi;     /* incidence direction                          */
+    double E0;             /* energy (only scales V and lambdabar)         */
+    double cap_U0;         /* ABC U0                                       */
+    double cap_alpha;      /* ABC alpha                                    */
+    long   iterations;     /* Iter                                         */
+    char   prefix[256];    /* surface file name prefix                     */
+    int    precision;      /* PSTD_FP64 or PSTD_FP32                       */
+} pstd_params;
+
+/* Global geometry derived from pstd_params; the index maths of
+ * CPSTD_QScat::DomainDecomposition (PSTD_QScat.cpp:270-396) and the
+ * injection corner of InitTS (PSTD_QScat.cpp:619-653).  All indices are
+ * global and inclusive. */
+typedef struct pstd_geometry {
+    long N[3];                       /* global grid                        */
+    long abc1[3], abc2[3];           /* m_X1ABC / m_X2ABC ...              */
+    long ksi1a[3], ksi1b[3];         /* m_X1aKsi / m_X1bKsi ...            */
+    long ksi2a[3], ksi2b[3];         /* m_X2aKsi / m_X2bKsi ...            */
+    long vrtl1[3], vrtl2[3];         /* m_X1Vrtl / m_X2Vrtl ...            */
+    long vbox1[3], vbox2[3];         /* potential box (union of m_x1V..)   */
+    long inj[3];                     /* m_XInj m_YInj m_ZInj               */
+    double orig[3];                  /* m_OrigX ...                        */
+    long n_seg[3];                   /* subdomains per axis                */
+    /* per axis a and subdomain r (r < n_seg[a] <= PSTD_MAX_SEG):        */
+    long fft1[3][64], fft2[3][64];   /* m_x1FFT / m_x2FFT                  */
+    long core1[3][64], core2[3][64]; /* m_x1Core / m_x2Core                */
+} pstd_geometry;
+#define PSTD_MAX_SEG 64
+
+typedef struct pstd_solver pstd_solver;
+
+/* Potential hook; run_environment.h:82, potential.cpp:39.  Scaled,
+ * centre-relative coordinates in; energy in E0 units out.  Called on the host
+ * during pstd_create, possibly from several threads: it must be reentrant. */
+typedef double (*pstd_potential_fn)(double x, double y, double z);
+
+/* ---- input file and geometry ---------------------------------------- */
+
+/* Parse a reference-format input file (qscat.cpp:214-297 load_data and
+ * utility.cpp:108-120 getaline).  Returns 3 if the file cannot be opened, 4
+ * if fewer than ten fields are present.  precision is set to PSTD_FP64. */
+int pstd_load_input(const char *path, pstd_params *out);
+
+/* Decomposition tables (PSTD_QScat.cpp:270-396, :619-653).  Returns 22 on an
+ * inconsistent geometry (see pstd_create). */
+int pstd_compute_geometry(const pstd_params *p, pstd_geometry *out);
+
+/* ---- solver lifecycle ----------------------------------------------- */
+
+/* CPSTD_QScat::Init (PSTD_QScat.cpp:108-236) with a potential hook: samples
+ * V on the potential box (InitV, :1121-1141), builds every coefficient table
+ * (InitFFT :496, InitABC :560, InitTS :617), sets the initial incident field
+ * (InitPsi :959) and clears the surface accumulators (:1143).  The solver
+ * owns all device memory.  Errors: 22 invalid geometry (h < 9 with more than
+ * one subdomain on an axis, a core shorter than h-1, a virtual surface whose
+ * 8th-order stencil leaves the local line, ...), 12 out of device memory,
+ * 19 no gfx950 device. */
+int pstd_create(const pstd_params *p, pstd_potential_fn V, pstd_solver **out);
+
+/* As pstd_create, but V is given already sampled on the global potential
+ * box [vbox1, vbox2] (x slowest, z fastest), in E0 units.  nvals must equal
+ * the box size. */
+int pstd_create_vbox(const pstd_params *p, const double *vbox, size_t nvals,
+                     pstd_solver **out);
+
+/* ~CPSTD_QScat (PSTD_QScat.cpp:2566-2641). */
+void pstd_destroy(pstd_solver *s);
+
+/* ---- time stepping -------------------------------------------------- */
+
+/* CPSTD_QScat::Update repeated nsteps times (PSTD_QScat.cpp:1233-1446):
+ * three windowed local-Fourier Laplacians, potential with its one-period
+ * ramp, TF/SF source, ABC mask, swap; the overlap exchange (:1539-1731) is
+ * implicit in the windowed line loads.  Asynchronous on the solver's
+ * stream. */
+int pstd_update(pstd_solver *s, long nsteps);
+
+/* CPSTD_QScat::IterationsToAccumulateSurfaceTerms (PSTD_QScat.cpp:1733). */
+long pstd_steps_per_period(const pstd_solver *s);
+
+/* CPSTD_QScat::AccumulateSurfaceTerms (PSTD_QScat.cpp:1738-2052) on the
+ * current psi. */
+int pstd_accumulate_surface_terms(pstd_solver *s);
+
+/* Update + AccumulateSurfaceTerms, nsteps times (the driver's accumulation
+ * window, qscat.cpp:191-192). */
+int pstd_update_accumulate(pstd_solver *s, long nsteps);
+
+/* Number of accumulations since the last reset (m_nSur) and the time index
+ * of the newest psi (m_nTau). */
+long pstd_surface_count(const pstd_solver *s);
+long pstd_time_index(const pstd_solver *s);
+
+/* Copy the finalized surface terms (accumulators / m_nSur,
+ * FinalizeSurfaceTerms PSTD_QScat.cpp:2054-2154) into out, in the body
+ * layout of the surface file (PSTD_QScat.cpp:2464-2552): x1 psi, x1 dpsi,
+ * x2 psi, x2 dpsi, each [Ny][Nz] complex; y1,y1',y2,y2' [Nx][Nz]; z1,z1',z2,
+ * z2' [Nx][Ny].  deriv selects PS or FD.  Non-destructive.  nvals must be
+ * 2*(4 Ny Nz + 4 Nx Nz + 4 Nx Ny) doubles. */
+int pstd_get_surfaces(pstd_solver *s, int deriv, double *out, size_t nvals);
+
+/* CPSTD_QScat::SaveVirtualSurfaces (PSTD_QScat.cpp:2390-2404): finalize,
+ * write <prefix>_PS_<n>.dat and <prefix>_FD_<n>.dat with the reference's
+ * header (int precision, 10 long, 7 QPrecision) and body, then reset.
+ * With no accumulation it prints the reference's warning and writes
+ * nothing. */
+int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix);
+
+/* Zero the accumulators (ResetSurfaceTerms, PSTD_QScat.cpp:2156-2189). */
+int pstd_reset_surface_terms(pstd_solver *s);
+
+/* ---- state access (tests, warm start, diagnostics) ------------------ */
+
+/* Copy psi^n (level 0) or psi^(n-1) (level 1) of the whole global grid to
+ * host as interleaved complex doubles, [Nx][Ny][Nz]. */
+int pstd_get_psi(pstd_solver *s, int level, double *out, size_t nvals);
+
+/* Overwrite psi^n (level 0) or psi^(n-1) (level 1) with host data of the
+ * same layout. */
+int pstd_set_psi(pstd_solver *s, int level, const double *in, size_t nvals);
+
+/* Set the time index m_nTau of the newest psi (used with pstd_set_psi). */
+int pstd_set_time_index(pstd_solver *s, long ntau);
+
+/* Block until all queued work of the solver has finished. */
+int pstd_synchronize(pstd_solver *s);
+
+/* Last error message of this thread ("" if none). */
+const char *pstd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSTD_H */

@@ -1,0 +1,1123 @@
+/*
+ * pstd_oracle.cpp -- CPU oracle for the PSTD_QScat time-stepping core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
+ * tests, by the smoke check and by the CPU-baseline leg of the benchmark,
+ * and by nothing in the product library.
+ *
+ * It restates, in FP64 C++ with OpenMP, the algorithm of the reference
+ * (UnitedTabby/PSTD-ScalarWaveQuantumScattering) the way the reference
+ * organises it: one local box per MPI rank with halos, an explicit overlap
+ * exchange between the boxes after every step, and one complex FFT per local
+ * line.  The MPI ranks become an array of boxes in one process; MPI_Sendrecv
+ * becomes a copy between boxes.  MKL DFTI is replaced by the DFT definition
+ * (forward e^{-2 pi i jk/n} unscaled, backward e^{+} scaled 1/n; the DFTI
+ * settings of PSTD_QScat.cpp:502-505), computed by the mixed-radix FFT below.
+ *
+ * Each function cites the reference file:line it follows.  The GPU library
+ * uses a different formulation (one global array, windowed dense line
+ * operators), so agreement between the two is a real check.
+ *
+ * Parity pins (tests/test_oracle_*.py): ksi family and k tables against
+ * closed forms and numerical derivatives; FFT against numpy.fft; decomposition
+ * against integer tables; exchange against the window invariant; one step
+ * against a dense brute-force restatement in numpy; TF/SF source against the
+ * analytic (lap Xi + 2i k.grad Xi) e^{ik.r} of the closed-form profile;
+ * surface capture against a synthetic plane wave; file bytes against the
+ * reference's header layout.  No oracle function is left unpinned.
+ */
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+#include "../include/pstd.h"
+
+typedef std::complex<double> cplx;
+
+namespace {
+
+/* run_environment.h:56-70 (FP64 branch) */
+const double TINY = 1.0e-10;
+const double PI = 3.141592653589793238462643383279502884197;
+const double TWOPI = 6.283185307179586476925286766559005768394;
+const double FOURPI = 12.56637061435917295385057353311801153679;
+const double SIXPI = 18.84955592153875943077586029967701730518;
+const double MICROEV2NMBAR = 4.552059563;
+
+/* utility.cpp:32-45: SMOOTH4 coefficients (default build) and the
+ * integrated Blackman-Harris set selected by __IBH__. */
+struct Profile { double a0, a1, a2, a3; };
+const Profile SMOOTH4 = {1.0, -4.0 / 3.0, 1.0 / 3.0, 0.0};
+const Profile IBH = {1.0, -1.3611, 0.3938, -0.0326};
+Profile g_prof = SMOOTH4;
+
+/* utility.cpp:47-58 */
+double ksi(double x) {
+    if (x < TINY) return 0.0;
+    if (x > 1.0 - TINY) return 1.0;
+    return g_prof.a0 * x + (g_prof.a1 / TWOPI) * std::sin(TWOPI * x) +
+           (g_prof.a2 / FOURPI) * std::sin(FOURPI * x) +
+           (g_prof.a3 / SIXPI) * std::sin(SIXPI * x);
+}
+/* utility.cpp:60-68 */
+double dksi(double x) {
+    if (x < TINY || x > 1.0 - TINY) return 0.0;
+    return g_prof.a0 + g_prof.a1 * std::cos(TWOPI * x) + g_prof.a2 * std::cos(FOURPI * x) +
+           g_prof.a3 * std::cos(SIXPI * x);
+}
+/* utility.cpp:70-78 */
+double d2ksi(double x) {
+    if (x < TINY || x > 1.0 - TINY) return 0.0;
+    return -TWOPI * (g_prof.a1 * std::sin(TWOPI * x) + 2.0 * g_prof.a2 * std::sin(FOURPI * x) +
+                     3.0 * g_prof.a3 * std::sin(SIXPI * x));
+}
+/* utility.cpp:95-106 (identical to ksi) */
+double weight(double x) { return ksi(x); }
+
+/* utility.cpp:80-93 */
+void kcoeff(double *k1, double *k2, double dD, long nD) {
+    for (long i = 0; i < nD; ++i) {
+        double kk;
+        if (i <= (nD >> 1))
+            kk = TWOPI / dD * (double)i / (double)nD;
+        else
+            kk = TWOPI / dD * ((double)i / (double)nD - 1.0);
+        k1[i * 2] = -kk * std::sin(kk * dD / 2.0);
+        k1[i * 2 + 1] = kk * std::cos(kk * dD / 2.0);
+        k2[i] = kk * kk;
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * Mixed-radix complex FFT (recursive decimation in time).  Twiddles are
+ * computed once per length in long double.  Forward: unscaled, e^{-}.
+ * Backward: e^{+}, scaled by 1/n.  This replaces DftiComputeForward /
+ * DftiComputeBackward with the settings of PSTD_QScat.cpp:502-505.
+ * ---------------------------------------------------------------------- */
+struct FFT {
+    long n = 0;
+    std::vector<int> rad;
+    std::vector<cplx> w;  // w[t] = exp(-2 pi i t / n)
+
+    void plan(long len) {
+        n = len;
+        rad.clear();
+        long m = len;
+        while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+        while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+        for (long p = 3; m > 1; p += 2)
+            while (m % p == 0) { rad.push_back((int)p); m /= p; }
+        w.resize(n);
+        for (long t = 0; t < n; ++t) {
+            long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+            w[t] = cplx((double)cosl(a), (double)sinl(a));
+        }
+    }
+    // out[0..nsub) = DFT_nsub of in[0], in[stride], ..., sign via conj flag
+    void rec(const cplx *in, long stride, cplx *out, long nsub, int lev, bool inv) const {
+        if (nsub == 1) { out[0] = in[0]; return; }
+        const int p = rad[lev];
+        const long m = nsub / p;
+        for (int q = 0; q < p; ++q) rec(in + q * stride, stride * p, out + q * m, m, lev + 1, inv);
+        const long tw = n / nsub;   // twiddle step for omega_nsub
+        const long wp = n / p;      // omega_p = w[wp]
+        std::vector<cplx> t(p), y(p);
+        for (long k = 0; k < m; ++k) {
+            for (int q = 0; q < p; ++q) {
+                cplx v = out[q * m + k];
+                long e = (q * k * tw) % n;
+                cplx ww = inv ? std::conj(w[e]) : w[e];
+                t[q] = (q == 0) ? v : v * ww;
+            }
+            if (p == 2) {
+                y[0] = t[0] + t[1];
+                y[1] = t[0] - t[1];
+            } else if (p == 4) {
+                cplx a = t[0] + t[2], b = t[0] - t[2], c = t[1] + t[3], d = t[1] - t[3];
+                // multiply d by -i (forward) or +i (inverse)
+                cplx di = inv ? cplx(-d.imag(), d.real()) : cplx(d.imag(), -d.real());
+                y[0] = a + c;
+                y[2] = a - c;
+                y[1] = b + di;
+                y[3] = b - di;
+            } else {
+                for (int s = 0; s < p; ++s) {
+                    cplx acc = 0.0;
+                    for (int q = 0; q < p; ++q) {
+                        long e = ((long)((q * s) % p)) * wp;
+                        cplx ww = inv ? std::conj(w[e]) : w[e];
+                        acc += t[q] * ww;
+                    }
+                    y[s] = acc;
+                }
+            }
+            for (int s = 0; s < p; ++s) out[k + m * s] = y[s];
+        }
+    }
+    void forward(cplx *data, cplx *tmp) const {
+        rec(data, 1, tmp, n, 0, false);
+        std::memcpy(data, tmp, sizeof(cplx) * n);
+    }
+    void backward(cplx *data, cplx *tmp) const {
+        rec(data, 1, tmp, n, 0, true);
+        const double s = 1.0 / (double)n;
+        for (long i = 0; i < n; ++i) data[i] = tmp[i] * s;
+    }
+};
+
+struct Slab { long xa, xb, ya, yb, za, zb, nx, ny, nz; };
+
+/* One MPI rank of the reference: its coordinates, local ranges and local
+ * arrays (PSTD_QScat.h:112-180). */
+struct Rank {
+    int c[3];
+    long f1[3], f2[3], c1[3], c2[3];   // m_x1FFT.., m_x1Core.. (global)
+    long i1[3], i2[3], nc[3];          // m_i1Core.., m_nxCore..
+    long nx, ny, nz, nyz;              // local box, nz = 2*Lz
+    std::vector<double> psi0, psi;     // m_Psi0, m_Psi
+    std::vector<double> g[3];          // m_g1..m_g3
+    bool flgKsi[6];
+    Slab slab[6];
+    std::vector<double> inc[6];        // m_Inc_x1 .. m_Inc_z2, [nx][ny][2nz]
+    bool flgV;
+    long v1[3], v2[3], nV[3];
+    std::vector<double> V;             // m_V_i
+    bool flgVrtl[6];
+    std::vector<double> sPsi[6], sPS[6], sFD[6];  // local surface planes
+    long sN[6][2];                     // local plane dims (complex)
+};
+
+struct Oracle {
+    pstd_params p;
+    long N[3];
+    long abc1[3], abc2[3], k1a[3], k1b[3], k2a[3], k2b[3], vr1[3], vr2[3], inj[3];
+    double orig[3];
+    double alpha2;                     // m_alpha_2dtau_i
+    double wgt[PSTD_NWEIGHT];
+    std::vector<double> k1[3], k2[3];
+    FFT fft[3];
+    std::vector<Rank> ranks;
+    long nTau, nSur;
+    std::string err;
+};
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+
+/* PSTD_QScat.cpp:277-331 for one axis. */
+void axis_ranges(long P, long L, long h, long r, long &c1, long &c2, long &f1, long &f2) {
+    if (P == 1) {
+        c1 = f1 = 0;
+        c2 = f2 = L - 1;
+    } else if (r == 0) {
+        c1 = f1 = 0;
+        f2 = f1 + L - 1;
+        c2 = f2 - h;
+    } else {
+        c1 = r * (L - 2 * h) + h;
+        f1 = c1 - h;
+        f2 = f1 + L - 1;
+        c2 = (r == P - 1) ? f2 : (f2 - h);
+    }
+}
+
+inline double &at(std::vector<double> &a, const Rank &R, long i, long j, long k2) {
+    return a[i * R.nyz + j * R.nz + k2];
+}
+
+/* ----------------------------------------------------------------------
+ * Init chain, PSTD_QScat.cpp:108-236
+ * -------------------------------------------------------------------- */
+void decompose(Oracle &O) {
+    const pstd_params &p = O.p;
+    for (int a = 0; a < 3; ++a) {
+        O.N[a] = p.nprocs[a] * (p.L[a] - 2 * p.n_halo) + 2 * p.n_halo;  // :277
+        O.abc1[a] = p.n_abc - 1;                                           // :334
+        O.abc2[a] = O.N[a] - p.n_abc;                                      // :335
+        O.k1a[a] = O.abc1[a] + p.n_sca + 1;                                // :342
+        O.k1b[a] = O.k1a[a] + p.n_ksi - 1;                                 // :343
+        O.k2b[a] = O.abc2[a] - p.n_sca - 1;                                // :344
+        O.k2a[a] = O.k2b[a] - p.n_ksi + 1;                                 // :345
+        O.vr1[a] = O.abc1[a] + (p.n_sca + 1) / 2;                          // :390
+        O.vr2[a] = O.abc2[a] - (p.n_sca + 1) / 2;                          // :391
+        O.orig[a] = (double)(O.N[a] - 1) / 2.0;                            // :138
+    }
+    int P0 = p.nprocs[0], P1 = p.nprocs[1], P2 = p.nprocs[2];
+    O.ranks.resize((size_t)P0 * P1 * P2);
+    int idx = 0;
+    for (int x = 0; x < P0; ++x)
+        for (int y = 0; y < P1; ++y)
+            for (int z = 0; z < P2; ++z, ++idx) {
+                Rank &R = O.ranks[idx];
+                R.c[0] = x; R.c[1] = y; R.c[2] = z;
+                for (int a = 0; a < 3; ++a) {
+                    axis_ranges(p.nprocs[a], p.L[a], p.n_halo, R.c[a], R.c1[a], R.c2[a], R.f1[a], R.f2[a]);
+                    R.nc[a] = R.c2[a] - R.c1[a] + 1;
+                    R.i1[a] = R.c1[a] - R.f1[a];
+                    R.i2[a] = R.c2[a] - R.f1[a];
+                }
+                R.nx = p.L[0];
+                R.ny = p.L[1];
+                R.nz = p.L[2] * 2;
+                R.nyz = R.ny * R.nz;
+                R.psi0.assign((size_t)R.nx * R.nyz, 0.0);
+                R.psi.assign((size_t)R.nx * R.nyz, 0.0);
+                // potential box, :368-387
+                if (R.c1[0] >= O.k2a[0] || R.c2[0] <= O.k1b[0] || R.c1[1] >= O.k2a[1] ||
+                    R.c2[1] <= O.k1b[1] || R.c1[2] >= O.k2a[2] || R.c2[2] <= O.k1b[2]) {
+                    R.flgV = false;
+                } else {
+                    R.flgV = true;
+                    for (int a = 0; a < 3; ++a) {
+                        R.v1[a] = (R.c1[a] > O.k1b[a]) ? R.c1[a] : (O.k1b[a] + 1);
+                        R.v2[a] = (R.c2[a] < O.k2a[a]) ? R.c2[a] : (O.k2a[a] - 1);
+                        R.nV[a] = R.v2[a] - R.v1[a] + 1;
+                    }
+                    R.V.assign((size_t)R.nV[0] * R.nV[1] * R.nV[2], 0.0);
+                }
+            }
+}
+
+/* PSTD_QScat.cpp:560-608 */
+void init_abc(Oracle &O) {
+    for (Rank &R : O.ranks)
+        for (int a = 0; a < 3; ++a) {
+            R.g[a].assign(R.nc[a], 0.0);
+            for (long i = R.c1[a]; i <= R.c2[a]; ++i) {
+                double tmp;
+                if (i < (O.N[a] >> 1))
+                    tmp = std::cosh(O.p.cap_alpha * (double)i);
+                else
+                    tmp = std::cosh(O.p.cap_alpha * (double)(O.N[a] - 1 - i));
+                R.g[a][i - R.c1[a]] = 1.0 - O.p.dtau * O.p.cap_U0 / (tmp * tmp);
+            }
+        }
+}
+
+/* PSTD_QScat.cpp:619-653: injection corner from the octant of (theta,phi). */
+void injection_corner(Oracle &O) {
+    double ct = std::cos(O.p.theta), sp = std::sin(O.p.phi), cp = std::cos(O.p.phi);
+    O.inj[0] = (cp >= 0.0) ? O.k1a[0] : O.k2b[0];
+    O.inj[1] = (sp >= 0.0) ? O.k1a[1] : O.k2b[1];
+    O.inj[2] = (ct >= 0.0) ? O.k1a[2] : O.k2b[2];
+}
+
+/* PSTD_QScat.cpp:617-955 */
+void init_ts(Oracle &O) {
+    const pstd_params &p = O.p;
+    const double ct = std::cos(p.theta), st = std::sin(p.theta);
+    const double cph = std::cos(p.phi), sph = std::sin(p.phi);
+    const long i0 = O.inj[0], j0 = O.inj[1], k0 = O.inj[2];
+    const double dx = p.d[0] * st * cph, dy = p.d[1] * st * sph, dz = p.d[2] * ct;  // :656-658
+    const double n0t1 = (double)(p.n_ksi - 1);                                       // :773
+    const long X1a = O.k1a[0], X1b = O.k1b[0], X2a = O.k2a[0], X2b = O.k2b[0];
+    const long Y1a = O.k1a[1], Y1b = O.k1b[1], Y2a = O.k2a[1], Y2b = O.k2b[1];
+    const long Z1a = O.k1a[2], Z1b = O.k1b[2], Z2a = O.k2a[2], Z2b = O.k2b[2];
+
+    for (Rank &R : O.ranks) {
+        for (int s = 0; s < 6; ++s) R.flgKsi[s] = false;
+        const long x1 = R.c1[0], x2 = R.c2[0], y1 = R.c1[1], y2 = R.c2[1], z1 = R.c1[2], z2 = R.c2[2];
+        // :665-668 non-crossing situations excluded
+        if (x2 < X1a || x1 > X2b || y2 < Y1a || y1 > Y2b || z2 < Z1a || z1 > Z2b ||
+            (x1 > X1b && x2 < X2a && y1 > Y1b && y2 < Y2a && z1 > Z1b && z2 < Z2a))
+            continue;
+        R.flgKsi[0] = (x1 <= X1b);  // :670-675
+        R.flgKsi[1] = (x2 >= X2a);
+        R.flgKsi[2] = (y1 <= Y1b);
+        R.flgKsi[3] = (y2 >= Y2a);
+        R.flgKsi[4] = (z1 <= Z1b);
+        R.flgKsi[5] = (z2 >= Z2a);
+        // slab extents :677-771; the slab-normal range is the transition
+        // layer, the other two the whole TF box, all clipped to the core.
+        for (int s = 0; s < 6; ++s) {
+            if (!R.flgKsi[s]) continue;
+            Slab &S = R.slab[s];
+            long lo[3] = {X1a, Y1a, Z1a}, hi[3] = {X2b, Y2b, Z2b};
+            int a = s / 2;
+            if (s % 2 == 0) { lo[a] = O.k1a[a]; hi[a] = O.k1b[a]; }
+            else            { lo[a] = O.k2a[a]; hi[a] = O.k2b[a]; }
+            S.xa = (x1 > lo[0]) ? x1 : lo[0];
+            S.xb = (x2 < hi[0]) ? x2 : hi[0];
+            S.ya = (y1 > lo[1]) ? y1 : lo[1];
+            S.yb = (y2 < hi[1]) ? y2 : hi[1];
+            S.za = (z1 > lo[2]) ? z1 : lo[2];
+            S.zb = (z2 < hi[2]) ? z2 : hi[2];
+            S.nx = S.xb - S.xa + 1;
+            S.ny = S.yb - S.ya + 1;
+            S.nz = (S.zb - S.za + 1) * 2;
+            R.inc[s].assign((size_t)(S.nx > 0 ? S.nx : 0) * (S.ny > 0 ? S.ny : 0) * (S.nz > 0 ? S.nz : 0), 0.0);
+        }
+        // coefficients :773-954.  Array layout [i][j][2k] for every slab.
+        for (int s = 0; s < 2; ++s) {  // x1 (:774-804), x2 (:806-834)
+            if (!R.flgKsi[s]) continue;
+            Slab &S = R.slab[s];
+            if (S.nx <= 0 || S.ny <= 0 || S.nz <= 0) continue;
+            double l = n0t1 * p.d[0];
+            double coef = 2.0 * p.dtau / l;
+            long nz = S.nz, nyz = S.ny * nz;
+            for (long j = S.ya; j <= S.yb; ++j) {
+                double rj = (j - j0) * dy;
+                double tmp1 = ksi((double)(j - Y1a) / n0t1) * ksi((double)(Y2b - j) / n0t1) * coef;
+                for (long i = S.xa; i <= S.xb; ++i) {
+                    double rij = rj + (i - i0) * dx;
+                    double x = (s == 0) ? (double)(i - X1a) / n0t1 : (double)(X2b - i) / n0t1;
+                    double tmp_r = tmp1 * d2ksi(x) / l;
+                    double tmp_i = tmp1 * 2.0 * dksi(x) * st * cph;
+                    if (s == 1) tmp_i = -tmp_i;   // :820
+                    double *pInc = R.inc[s].data() + (i - S.xa) * nyz + (j - S.ya) * nz;
+                    for (long k = S.za; k <= S.zb; ++k) {
+                        double r = rij + (k - k0) * dz;
+                        double cr = std::cos(r), sr = std::sin(r);
+                        double tmp2 = ksi((double)(k - Z1a) / n0t1) * ksi((double)(Z2b - k) / n0t1);
+                        pInc[(k - S.za) * 2] = tmp2 * (tmp_r * cr - tmp_i * sr);
+                        pInc[(k - S.za) * 2 + 1] = tmp2 * (tmp_r * sr + tmp_i * cr);
+                    }
+                }
+            }
+        }
+        for (int s = 2; s < 4; ++s) {  // y1 (:836-864), y2 (:866-894)
+            if (!R.flgKsi[s]) continue;
+            Slab &S = R.slab[s];
+            if (S.nx <= 0 || S.ny <= 0 || S.nz <= 0) continue;
+            double l = n0t1 * p.d[1];
+            double coef = 2.0 * p.dtau / l;
+            long nz = S.nz, nyz = S.ny * nz;
+            for (long i = S.xa; i <= S.xb; ++i) {
+                double ri = (i - i0) * dx;
+                double tmp1 = ksi((double)(i - X1a) / n0t1) * ksi((double)(X2b - i) / n0t1) * coef;
+                for (long j = S.ya; j <= S.yb; ++j) {
+                    double rij = ri + (j - j0) * dy;
+                    double y = (s == 2) ? (double)(j - Y1a) / n0t1 : (double)(Y2b - j) / n0t1;
+                    double tmp_r = tmp1 * d2ksi(y) / l;
+                    double tmp_i = tmp1 * 2.0 * dksi(y) * st * sph;
+                    if (s == 3) tmp_i = -tmp_i;   // :880
+                    double *pInc = R.inc[s].data() + (i - S.xa) * nyz + (j - S.ya) * nz;
+                    for (long k = S.za; k <= S.zb; ++k) {
+                        double r = rij + (k - k0) * dz;
+                        double cr = std::cos(r), sr = std::sin(r);
+                        double tmp2 = ksi((double)(k - Z1a) / n0t1) * ksi((double)(Z2b - k) / n0t1);
+                        pInc[(k - S.za) * 2] = tmp2 * (tmp_r * cr - tmp_i * sr);
+                        pInc[(k - S.za) * 2 + 1] = tmp2 * (tmp_r * sr + tmp_i * cr);
+                    }
+                }
+            }
+        }
+        for (int s = 4; s < 6; ++s) {  // z1 (:896-924), z2 (:926-954)
+            if (!R.flgKsi[s]) continue;
+            Slab &S = R.slab[s];
+            if (S.nx <= 0 || S.ny <= 0 || S.nz <= 0) continue;
+            double l = n0t1 * p.d[2];
+            double coef = 2.0 * p.dtau / l;
+            long nz = S.nz, nyz = S.ny * nz;
+            for (long i = S.xa; i <= S.xb; ++i) {
+                double ri = (i - i0) * dx;
+                double tmp1 = ksi((double)(i - X1a) / n0t1) * ksi((double)(X2b - i) / n0t1) * coef;
+                for (long k = S.za; k <= S.zb; ++k) {
+                    double rik = ri + (k - k0) * dz;
+                    double z = (s == 4) ? (double)(k - Z1a) / n0t1 : (double)(Z2b - k) / n0t1;
+                    double tmp_r = tmp1 * d2ksi(z) / l;
+                    double tmp_i = tmp1 * 2.0 * dksi(z) * ct;
+                    if (s == 5) tmp_i = -tmp_i;   // :940
+                    double *pInc = R.inc[s].data() + (i - S.xa) * nyz + (k - S.za) * 2;
+                    for (long j = S.ya; j <= S.yb; ++j) {
+                        double r = rik + (j - j0) * dy;
+                        double cr = std::cos(r), sr = std::sin(r);
+                        double tmp2 = ksi((double)(j - Y1a) / n0t1) * ksi((double)(Y2b - j) / n0t1);
+                        pInc[(j - S.ya) * nz] = tmp2 * (tmp_r * cr - tmp_i * sr);
+                        pInc[(j - S.ya) * nz + 1] = tmp2 * (tmp_r * sr + tmp_i * cr);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ExchangeData, PSTD_QScat.cpp:1539-1731, with every MPI_Sendrecv pair
+ * turned into a copy between neighbouring boxes (send offsets :479-492,
+ * datatypes :401-477: h-1 planes, core range of the other two axes). */
+void exchange(Oracle &O) {
+    const long h = O.p.n_halo, h1 = h - 1;
+    const int P[3] = {O.p.nprocs[0], O.p.nprocs[1], O.p.nprocs[2]};
+    auto rid = [&](int x, int y, int z) { return ((size_t)x * P[1] + y) * P[2] + z; };
+    for (int a = 0; a < 3; ++a) {
+        if (P[a] <= 1) continue;
+        // copies: for every neighbour pair (lo, hi) along a
+        #pragma omp parallel for schedule(dynamic)
+        for (long q = 0; q < (long)O.ranks.size(); ++q) {
+            Rank &Lo = O.ranks[q];
+            if (Lo.c[a] == P[a] - 1) continue;
+            int cc[3] = {Lo.c[0], Lo.c[1], Lo.c[2]};
+            cc[a] += 1;
+            Rank &Hi = O.ranks[rid(cc[0], cc[1], cc[2])];
+            int b = (a + 1) % 3, c = (a + 2) % 3;
+            for (long m = 0; m < h1; ++m) {
+                for (long u = 0; u < Lo.nc[b]; ++u)
+                    for (long v = 0; v < Lo.nc[c]; ++v) {
+                        // Lo's top core planes -> Hi's planes 1..h-1  (xb send -> xa recv)
+                        long s_idx[3], d_idx[3];
+                        s_idx[a] = Lo.i2[a] - (h - 2) + m;  d_idx[a] = 1 + m;
+                        s_idx[b] = Lo.i1[b] + u;            d_idx[b] = Hi.i1[b] + u;
+                        s_idx[c] = Lo.i1[c] + v;            d_idx[c] = Hi.i1[c] + v;
+                        for (int e = 0; e < 2; ++e)
+                            at(Hi.psi, Hi, d_idx[0], d_idx[1], d_idx[2] * 2 + e) =
+                                at(Lo.psi, Lo, s_idx[0], s_idx[1], s_idx[2] * 2 + e);
+                        // Hi's bottom core planes -> Lo's planes i2Core+1..  (xa send -> xb recv)
+                        s_idx[a] = Hi.i1[a] + m;            d_idx[a] = Lo.i2[a] + 1 + m;
+                        s_idx[b] = Hi.i1[b] + u;            d_idx[b] = Lo.i1[b] + u;
+                        s_idx[c] = Hi.i1[c] + v;            d_idx[c] = Lo.i1[c] + v;
+                        for (int e = 0; e < 2; ++e)
+                            at(Lo.psi, Lo, d_idx[0], d_idx[1], d_idx[2] * 2 + e) =
+                                at(Hi.psi, Hi, s_idx[0], s_idx[1], s_idx[2] * 2 + e);
+                    }
+            }
+        }
+        // weights on the overlap: plane 0 -> 0, planes 1..8 *= w[0..7];
+        // plane L-1 -> 0, planes L-9..L-2 *= w[7..0]  (:1554-1603 and y/z twins)
+        #pragma omp parallel for schedule(dynamic)
+        for (long q = 0; q < (long)O.ranks.size(); ++q) {
+            Rank &R = O.ranks[q];
+            int b = (a + 1) % 3, c = (a + 2) % 3;
+            long La = O.p.L[a];
+            auto scale_plane = [&](long pl, double f) {
+                for (long u = R.i1[b]; u <= R.i2[b]; ++u)
+                    for (long v = R.i1[c]; v <= R.i2[c]; ++v) {
+                        long id[3];
+                        id[a] = pl; id[b] = u; id[c] = v;
+                        double *ptr = &at(R.psi, R, id[0], id[1], id[2] * 2);
+                        if (f == 0.0) { ptr[0] = 0.0; ptr[1] = 0.0; }
+                        else { ptr[0] *= f; ptr[1] *= f; }
+                    }
+            };
+            if (R.c1[a] > R.f1[a]) {
+                scale_plane(0, 0.0);
+                for (long i = 1; i <= PSTD_NWEIGHT; ++i) scale_plane(i, O.wgt[i - 1]);
+            }
+            if (R.c2[a] < R.f2[a]) {
+                long i2 = La - 1;
+                scale_plane(i2, 0.0);
+                for (long i = i2 - PSTD_NWEIGHT; i < i2; ++i) scale_plane(i, O.wgt[i2 - 1 - i]);
+            }
+        }
+    }
+}
+
+/* PSTD_QScat.cpp:959-1118 */
+void init_psi(Oracle &O) {
+    const pstd_params &p = O.p;
+    const double ct = std::cos(p.theta), st = std::sin(p.theta);
+    const double cph = std::cos(p.phi), sph = std::sin(p.phi);
+    const long i0 = O.inj[0], j0 = O.inj[1], k0 = O.inj[2];
+    const double n0t1 = (double)(p.n_ksi - 1);
+    const double dx = p.d[0] * st * cph, dy = p.d[1] * st * sph, dz = p.d[2] * ct;
+    const double dtc_r = std::cos(p.dtau), dtc_i = -std::sin(p.dtau);
+    for (Rank &R : O.ranks) {
+        auto P0 = [&](long i, long j, long k, int e) -> double & {
+            return at(R.psi0, R, i - R.f1[0], j - R.f1[1], (k - R.f1[2]) * 2 + e);
+        };
+        auto P1 = [&](long i, long j, long k, int e) -> double & {
+            return at(R.psi, R, i - R.f1[0], j - R.f1[1], (k - R.f1[2]) * 2 + e);
+        };
+        long ia = std::max(R.c1[0], O.k1a[0]), ib = std::min(R.c2[0], O.k2b[0]);
+        long ja = std::max(R.c1[1], O.k1a[1]), jb = std::min(R.c2[1], O.k2b[1]);
+        long ka = std::max(R.c1[2], O.k1a[2]), kb = std::min(R.c2[2], O.k2b[2]);
+        for (long i = ia; i <= ib; ++i)
+            for (long j = ja; j <= jb; ++j)
+                for (long k = ka; k <= kb; ++k) {
+                    double r = (double)(i - i0) * dx + (double)(j - j0) * dy + (double)(k - k0) * dz;
+                    double cr = std::cos(r), sr = std::sin(r);
+                    P0(i, j, k, 0) = cr;
+                    P0(i, j, k, 1) = sr;
+                    P1(i, j, k, 0) = cr * dtc_r - sr * dtc_i;
+                    P1(i, j, k, 1) = cr * dtc_i + sr * dtc_r;
+                }
+        auto scale = [&](long i, long j, long k, double t) {
+            P0(i, j, k, 0) *= t; P0(i, j, k, 1) *= t;
+            P1(i, j, k, 0) *= t; P1(i, j, k, 1) *= t;
+        };
+        if (R.flgKsi[0]) {  // :1000-1019
+            long ibb = std::min(R.c2[0], O.k1b[0]);
+            for (long i = ia; i <= ibb; ++i) {
+                double t = ksi((double)(i - O.k1a[0]) / n0t1);
+                for (long j = ja; j <= jb; ++j) for (long k = ka; k <= kb; ++k) scale(i, j, k, t);
+            }
+        }
+        if (R.flgKsi[1]) {  // :1021-1038
+            long iaa = std::max(R.c1[0], O.k2a[0]);
+            for (long i = iaa; i <= ib; ++i) {
+                double t = ksi((double)(O.k2b[0] - i) / n0t1);
+                for (long j = ja; j <= jb; ++j) for (long k = ka; k <= kb; ++k) scale(i, j, k, t);
+            }
+        }
+        if (R.flgKsi[2]) {  // :1040-1057
+            long jbb = std::min(R.c2[1], O.k1b[1]);
+            for (long j = ja; j <= jbb; ++j) {
+                double t = ksi((double)(j - O.k1a[1]) / n0t1);
+                for (long i = ia; i <= ib; ++i) for (long k = ka; k <= kb; ++k) scale(i, j, k, t);
+            }
+        }
+        if (R.flgKsi[3]) {  // :1059-1076
+            long jaa = std::max(R.c1[1], O.k2a[1]);
+            for (long j = jaa; j <= jb; ++j) {
+                double t = ksi((double)(O.k2b[1] - j) / n0t1);
+                for (long i = ia; i <= ib; ++i) for (long k = ka; k <= kb; ++k) scale(i, j, k, t);
+            }
+        }
+        if (R.flgKsi[4]) {  // :1078-1095
+            long kbb = std::min(R.c2[2], O.k1b[2]);
+            for (long k = ka; k <= kbb; ++k) {
+                double t = ksi((double)(k - O.k1a[2]) / n0t1);
+                for (long i = ia; i <= ib; ++i) for (long j = ja; j <= jb; ++j) scale(i, j, k, t);
+            }
+        }
+        if (R.flgKsi[5]) {  // :1097-1114
+            long kaa = std::max(R.c1[2], O.k2a[2]);
+            for (long k = kaa; k <= kb; ++k) {
+                double t = ksi((double)(O.k2b[2] - k) / n0t1);
+                for (long i = ia; i <= ib; ++i) for (long j = ja; j <= jb; ++j) scale(i, j, k, t);
+            }
+        }
+    }
+    exchange(O);
+    O.nTau = 1;  // :1117
+}
+
+/* PSTD_QScat.cpp:1121-1141 */
+void init_v(Oracle &O, const std::function<double(double, double, double)> &V) {
+    for (Rank &R : O.ranks) {
+        if (!R.flgV) continue;
+        #pragma omp parallel for
+        for (long i = R.v1[0]; i <= R.v2[0]; ++i) {
+            double x = (i - O.orig[0]) * O.p.d[0];
+            for (long j = R.v1[1]; j <= R.v2[1]; ++j) {
+                double y = (j - O.orig[1]) * O.p.d[1];
+                for (long k = R.v1[2]; k <= R.v2[2]; ++k) {
+                    double z = (k - O.orig[2]) * O.p.d[2];
+                    R.V[((i - R.v1[0]) * R.nV[1] + (j - R.v1[1])) * R.nV[2] + (k - R.v1[2])] =
+                        V(x, y, z) * 2.0 * O.p.dtau / O.p.E0;
+                }
+            }
+        }
+    }
+}
+
+/* PSTD_QScat.cpp:1143-1231 */
+void init_surfaces(Oracle &O) {
+    O.nSur = 0;
+    for (Rank &R : O.ranks) {
+        for (int f = 0; f < 6; ++f) {
+            int a = f / 2;
+            long v = (f % 2 == 0) ? O.vr1[a] : O.vr2[a];
+            R.flgVrtl[f] = (R.c1[a] <= v && v <= R.c2[a]);
+            int b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+            R.sN[f][0] = R.nc[b];
+            R.sN[f][1] = R.nc[c];
+            if (R.flgVrtl[f]) {
+                size_t n = (size_t)R.nc[b] * R.nc[c] * 2;
+                R.sPsi[f].assign(n, 0.0);
+                R.sPS[f].assign(n, 0.0);
+                R.sFD[f].assign(n, 0.0);
+            }
+        }
+    }
+}
+
+/* ----------------------------------------------------------------------
+ * One step, PSTD_QScat.cpp:1233-1446
+ * -------------------------------------------------------------------- */
+struct Work {
+    std::vector<cplx> buf, tmp;
+};
+
+/* UpdateX/Y/Z, PSTD_QScat.cpp:1448-1537: gather the full local line,
+ * forward FFT, x k2, backward FFT, psi0 core += (alpha*Im u, -alpha*Re u). */
+void update_line(Oracle &O, Rank &R, int a, long u, long v, Work &W) {
+    const long La = O.p.L[a];
+    long stride = (a == 0) ? R.nyz : (a == 1) ? R.nz : 2;
+    long base = (a == 0) ? u * R.nz + v * 2 : (a == 1) ? u * R.nyz + v * 2 : u * R.nyz + v * R.nz;
+    const double *psi = R.psi.data() + base;
+    double *psi0 = R.psi0.data() + base;
+    cplx *pb = W.buf.data();
+    for (long i = 0; i < La; ++i) pb[i] = cplx(psi[i * stride], psi[i * stride + 1]);
+    O.fft[a].forward(pb, W.tmp.data());
+    const double *k2 = O.k2[a].data();
+    for (long i = 0; i < La; ++i) pb[i] *= k2[i];
+    O.fft[a].backward(pb, W.tmp.data());
+    for (long i = R.i1[a]; i <= R.i2[a]; ++i) {
+        psi0[i * stride] += O.alpha2 * pb[i].imag();
+        psi0[i * stride + 1] -= O.alpha2 * pb[i].real();
+    }
+}
+
+void update(Oracle &O) {
+    const pstd_params &p = O.p;
+    const double t = O.nTau * p.dtau;                 // :1235
+    const double fV = t / TWOPI;                      // :1236
+    const double Iinc_r = std::sin(t), Iinc_i = std::cos(t);  // :1238-1239
+    const long maxL = std::max(p.L[0], std::max(p.L[1], p.L[2]));
+    const int nth = omp_get_max_threads();
+    std::vector<Work> work(nth);
+    for (Work &W : work) { W.buf.resize(maxL); W.tmp.resize(maxL); }
+
+    // X, Y, Z Laplacians (:1244-1265), each as its own parallel sweep
+    for (int a = 0; a < 3; ++a) {
+        int b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+        // enumerate (rank, u, v) line triples
+        std::vector<long> off(O.ranks.size() + 1, 0);
+        for (size_t q = 0; q < O.ranks.size(); ++q)
+            off[q + 1] = off[q] + O.ranks[q].nc[b] * O.ranks[q].nc[c];
+        const long total = off.back();
+        #pragma omp parallel for schedule(static)
+        for (long g = 0; g < total; ++g) {
+            size_t q = std::upper_bound(off.begin(), off.end(), g) - off.begin() - 1;
+            Rank &R = O.ranks[q];
+            long loc = g - off[q];
+            long u = R.i1[b] + loc / R.nc[c];
+            long v = R.i1[c] + loc % R.nc[c];
+            update_line(O, R, a, u, v, work[omp_get_thread_num()]);
+        }
+    }
+
+    #pragma omp parallel for schedule(dynamic)
+    for (long q = 0; q < (long)O.ranks.size(); ++q) {
+        Rank &R = O.ranks[q];
+        // potential term with its one-period ramp (:1268-1303)
+        if (R.flgV) {
+            const double f = (fV < 0.997) ? fV : 1.0;
+            for (long i = R.v1[0]; i <= R.v2[0]; ++i)
+                for (long j = R.v1[1]; j <= R.v2[1]; ++j) {
+                    const double *pV = R.V.data() + ((i - R.v1[0]) * R.nV[1] + (j - R.v1[1])) * R.nV[2];
+                    double *ps = &at(R.psi, R, i - R.f1[0], j - R.f1[1], 0);
+                    double *p0 = &at(R.psi0, R, i - R.f1[0], j - R.f1[1], 0);
+                    for (long k = R.v1[2]; k <= R.v2[2]; ++k) {
+                        long kt = (k - R.f1[2]) * 2;
+                        double vv = (fV < 0.997) ? f * pV[k - R.v1[2]] : pV[k - R.v1[2]];
+                        p0[kt] += vv * ps[kt + 1];
+                        p0[kt + 1] -= vv * ps[kt];
+                    }
+                }
+        }
+        // TF/SF incident source, slabs x1 x2 y1 y2 z1 z2 (:1306-1419)
+        for (int s = 0; s < 6; ++s) {
+            if (!R.flgKsi[s]) continue;
+            const Slab &S = R.slab[s];
+            if (S.nx <= 0 || S.ny <= 0 || S.nz <= 0) continue;
+            long nz = S.nz, nyz = S.ny * nz;
+            for (long i = S.xa; i <= S.xb; ++i)
+                for (long j = S.ya; j <= S.yb; ++j) {
+                    const double *pInc = R.inc[s].data() + (i - S.xa) * nyz + (j - S.ya) * nz;
+                    double *p0 = &at(R.psi0, R, i - R.f1[0], j - R.f1[1], 0);
+                    for (long k = S.za; k <= S.zb; ++k) {
+                        long kt = (k - S.za) * 2;
+                        p0[(k - R.f1[2]) * 2] -= pInc[kt] * Iinc_r - pInc[kt + 1] * Iinc_i;
+                        p0[(k - R.f1[2]) * 2 + 1] -= pInc[kt] * Iinc_i + pInc[kt + 1] * Iinc_r;
+                    }
+                }
+        }
+        // ABC mask on the whole core (:1421-1435)
+        for (long i = R.i1[0]; i <= R.i2[0]; ++i)
+            for (long j = R.i1[1]; j <= R.i2[1]; ++j) {
+                double tmp = R.g[0][i - R.i1[0]] * R.g[1][j - R.i1[1]];
+                double *p0 = &at(R.psi0, R, i, j, 0);
+                for (long k = R.i1[2]; k <= R.i2[2]; ++k) {
+                    p0[k * 2] *= tmp * R.g[2][k - R.i1[2]];
+                    p0[k * 2 + 1] *= tmp * R.g[2][k - R.i1[2]];
+                }
+            }
+    }
+    // swap (:1441-1443), exchange (:1444), advance (:1445)
+    for (Rank &R : O.ranks) R.psi0.swap(R.psi);
+    exchange(O);
+    ++O.nTau;
+}
+
+/* PSTD_QScat.cpp:1738-2052 */
+const double FD1 = 4.0 / 5.0, FD2 = -1.0 / 5.0, FD3 = 4.0 / 105.0, FD4 = -1.0 / 280.0;  // :43-46
+
+void accumulate(Oracle &O) {
+    const double itr = std::cos(O.nTau * O.p.dtau), iti = std::sin(O.nTau * O.p.dtau);
+    const double psr = itr / 2.0, psi_ = iti / 2.0;
+    const long maxL = std::max(O.p.L[0], std::max(O.p.L[1], O.p.L[2]));
+    for (int a = 0; a < 3; ++a) {
+        const double fdr = itr / O.p.d[a], fdi = iti / O.p.d[a];
+        int b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+        for (Rank &R : O.ranks) {
+            bool f1 = R.flgVrtl[2 * a], f2 = R.flgVrtl[2 * a + 1];
+            if (!f1 && !f2) continue;
+            long stride = (a == 0) ? R.nyz : (a == 1) ? R.nz : 2;
+            #pragma omp parallel
+            {
+                std::vector<cplx> pf(maxL), tmp(maxL);
+                #pragma omp for schedule(static)
+                for (long uu = 0; uu < R.nc[b] * R.nc[c]; ++uu) {
+                    long u = R.i1[b] + uu / R.nc[c], v = R.i1[c] + uu % R.nc[c];
+                    long base = (a == 0) ? u * R.nz + v * 2 : (a == 1) ? u * R.nyz + v * 2 : u * R.nyz + v * R.nz;
+                    const double *psi = R.psi.data() + base;
+                    for (long i = 0; i < O.p.L[a]; ++i) pf[i] = cplx(psi[i * stride], psi[i * stride + 1]);
+                    O.fft[a].forward(pf.data(), tmp.data());
+                    const double *k1 = O.k1[a].data();
+                    for (long i = 0; i < O.p.L[a]; ++i) {
+                        double re = pf[i].real() * k1[i * 2] - pf[i].imag() * k1[i * 2 + 1];
+                        double im = pf[i].real() * k1[i * 2 + 1] + pf[i].imag() * k1[i * 2];
+                        pf[i] = cplx(re, im);
+                    }
+                    O.fft[a].backward(pf.data(), tmp.data());
+                    for (int side = 0; side < 2; ++side) {
+                        int f = 2 * a + side;
+                        if (!R.flgVrtl[f]) continue;
+                        long vr = side == 0 ? O.vr1[a] : O.vr2[a];
+                        long li = vr - R.f1[a];
+                        size_t o = ((u - R.i1[b]) * R.nc[c] + (v - R.i1[c])) * 2;
+                        const double *pp = psi + li * stride;
+                        R.sPsi[f][o] += pp[0] * itr - pp[1] * iti;
+                        R.sPsi[f][o + 1] += pp[0] * iti + pp[1] * itr;
+                        double tr = FD4 * (pp[4 * stride] - pp[-4 * stride]) + FD3 * (pp[3 * stride] - pp[-3 * stride]) +
+                                    FD2 * (pp[2 * stride] - pp[-2 * stride]) + FD1 * (pp[stride] - pp[-stride]);
+                        double ti = FD4 * (pp[4 * stride + 1] - pp[-4 * stride + 1]) +
+                                    FD3 * (pp[3 * stride + 1] - pp[-3 * stride + 1]) +
+                                    FD2 * (pp[2 * stride + 1] - pp[-2 * stride + 1]) +
+                                    FD1 * (pp[stride + 1] - pp[-stride + 1]);
+                        R.sFD[f][o] += tr * fdr - ti * fdi;
+                        R.sFD[f][o + 1] += tr * fdi + ti * fdr;
+                        double sr = pf[li - 1].real() + pf[li].real();
+                        double si = pf[li - 1].imag() + pf[li].imag();
+                        R.sPS[f][o] += sr * psr - si * psi_;
+                        R.sPS[f][o + 1] += sr * psi_ + si * psr;
+                    }
+                }
+            }
+        }
+    }
+    O.nSur++;
+}
+
+/* Assemble the global surface planes in file-body order (the MPI-IO
+ * subarray placement of PSTD_QScat.cpp:2464-2552), divided by m_nSur. */
+void gather_surfaces(Oracle &O, int deriv, std::vector<double> &out, bool finalize) {
+    const long Nx = O.N[0], Ny = O.N[1], Nz = O.N[2];
+    size_t planeN[3] = {(size_t)Ny * Nz * 2, (size_t)Nx * Nz * 2, (size_t)Nx * Ny * 2};
+    out.assign(4 * (planeN[0] + planeN[1] + planeN[2]), 0.0);
+    size_t off = 0;
+    double sc = (finalize && O.nSur > 0) ? 1.0 / (double)O.nSur : 1.0;
+    for (int a = 0; a < 3; ++a) {
+        int b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+        long Nc = O.N[c];
+        for (int side = 0; side < 2; ++side) {
+            int f = 2 * a + side;
+            for (int kind = 0; kind < 2; ++kind) {
+                double *dst = out.data() + off;
+                for (Rank &R : O.ranks) {
+                    if (!R.flgVrtl[f]) continue;
+                    const std::vector<double> &src = kind == 0 ? R.sPsi[f] : (deriv == PSTD_DERIV_PS ? R.sPS[f] : R.sFD[f]);
+                    for (long u = 0; u < R.nc[b]; ++u)
+                        for (long v = 0; v < R.nc[c]; ++v)
+                            for (int e = 0; e < 2; ++e)
+                                dst[((R.c1[b] + u) * Nc + (R.c1[c] + v)) * 2 + e] =
+                                    src[(u * R.nc[c] + v) * 2 + e] * sc;
+                }
+                off += planeN[a];
+            }
+        }
+    }
+}
+
+int write_file(Oracle &O, const std::string &name, int deriv) {
+    std::vector<double> body;
+    gather_surfaces(O, deriv, body, true);
+    FILE *fp = std::fopen(name.c_str(), "wb");
+    if (!fp) return fail(PSTD_EIO, "Cannot open output file " + name);
+    int precision = (int)sizeof(double);
+    std::fwrite(&precision, sizeof(int), 1, fp);
+    long hdr[10] = {O.N[0], O.N[1], O.N[2], O.vr1[0], O.vr2[0], O.vr1[1], O.vr2[1], O.vr1[2], O.vr2[2], O.p.n_abc};
+    std::fwrite(hdr, sizeof(long), 10, fp);
+    double dh[7] = {O.p.d[0], O.p.d[1], O.p.d[2], O.orig[0], O.orig[1], O.orig[2], O.p.E0};
+    std::fwrite(dh, sizeof(double), 7, fp);
+    std::fwrite(body.data(), sizeof(double), body.size(), fp);
+    std::fclose(fp);
+    return PSTD_OK;
+}
+
+void reset_surfaces(Oracle &O) {
+    O.nSur = 0;
+    for (Rank &R : O.ranks)
+        for (int f = 0; f < 6; ++f) {
+            std::fill(R.sPsi[f].begin(), R.sPsi[f].end(), 0.0);
+            std::fill(R.sPS[f].begin(), R.sPS[f].end(), 0.0);
+            std::fill(R.sFD[f].begin(), R.sFD[f].end(), 0.0);
+        }
+}
+
+int validate(const pstd_params *p) {
+    if (!p) return fail(PSTD_EINVAL, "null params");
+    for (int a = 0; a < 3; ++a) {
+        if (p->nprocs[a] < 1) return fail(PSTD_ETOPO, "subdomain count < 1");
+        if (p->L[a] < 2 || p->d[a] <= 0.0) return fail(PSTD_EINVAL, "bad L or spacing");
+        if (p->nprocs[a] > 1) {
+            if (p->n_halo < PSTD_NWEIGHT + 1) return fail(PSTD_EINVAL, "halo < 9");
+            if (p->L[a] - 2 * p->n_halo < 1 && p->nprocs[a] > 2) return fail(PSTD_EINVAL, "interior core empty");
+            if (p->L[a] - p->n_halo < p->n_halo - 1) return fail(PSTD_EINVAL, "edge core < h-1");
+            if (p->nprocs[a] > 2 && p->L[a] - 2 * p->n_halo < p->n_halo - 1)
+                return fail(PSTD_EINVAL, "interior core < h-1");
+        }
+    }
+    if (p->n_ksi < 2 || p->dtau <= 0.0 || p->E0 == 0.0) return fail(PSTD_EINVAL, "bad n_ksi/dtau/E0");
+    return PSTD_OK;
+}
+
+Oracle *build(const pstd_params *p, const std::function<double(double, double, double)> &V, int &rc) {
+    rc = validate(p);
+    if (rc) return nullptr;
+    Oracle *O = new Oracle();
+    O->p = *p;
+    O->alpha2 = 2.0 * std::sin(p->dtau);  // :134
+    for (long i = 0; i < PSTD_NWEIGHT; ++i) O->wgt[i] = weight((double)(i + 1) / (double)(PSTD_NWEIGHT + 1));  // :104-105
+    decompose(*O);
+    for (int a = 0; a < 3; ++a) {  // InitFFT :496-557
+        O->k1[a].assign(p->L[a] * 2, 0.0);
+        O->k2[a].assign(p->L[a], 0.0);
+        kcoeff(O->k1[a].data(), O->k2[a].data(), p->d[a], p->L[a]);
+        O->fft[a].plan(p->L[a]);
+    }
+    init_abc(*O);
+    injection_corner(*O);
+    init_ts(*O);
+    init_psi(*O);
+    init_v(*O, V);
+    init_surfaces(*O);
+    return O;
+}
+
+}  // namespace
+
+/* ======================================================================
+ * C entry points (ctypes)
+ * ==================================================================== */
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+void orc_set_profile(int ibh) { g_prof = ibh ? IBH : SMOOTH4; }
+double orc_ksi(double x) { return ksi(x); }
+double orc_dksi(double x) { return dksi(x); }
+double orc_d2ksi(double x) { return d2ksi(x); }
+double orc_weight(double x) { return weight(x); }
+void orc_kcoeff(double *k1, double *k2, double d, long n) { kcoeff(k1, k2, d, n); }
+
+/* sign -1: forward, unscaled; +1: backward, scaled 1/n.  data interleaved. */
+void orc_fft(double *data, long n, int sign) {
+    FFT f;
+    f.plan(n);
+    std::vector<cplx> buf(n), tmp(n);
+    for (long i = 0; i < n; ++i) buf[i] = cplx(data[2 * i], data[2 * i + 1]);
+    if (sign < 0) f.forward(buf.data(), tmp.data());
+    else f.backward(buf.data(), tmp.data());
+    for (long i = 0; i < n; ++i) { data[2 * i] = buf[i].real(); data[2 * i + 1] = buf[i].imag(); }
+}
+
+void *orc_create_vbox(const pstd_params *p, const double *vbox, long nvals, int *rc) {
+    // V box = (ksi1b, ksi2a) exclusive on every axis
+    long N[3], lo[3], n[3];
+    for (int a = 0; a < 3; ++a) {
+        N[a] = p->nprocs[a] * (p->L[a] - 2 * p->n_halo) + 2 * p->n_halo;
+        long k1b = (p->n_abc - 1) + p->n_sca + 1 + p->n_ksi - 1;
+        long k2a = (N[a] - p->n_abc) - p->n_sca - 1 - p->n_ksi + 1;
+        lo[a] = k1b + 1;
+        n[a] = k2a - 1 - lo[a] + 1;
+        if (n[a] < 0) n[a] = 0;
+    }
+    if (vbox && nvals != n[0] * n[1] * n[2]) {
+        *rc = fail(PSTD_EINVAL, "vbox size mismatch");
+        return nullptr;
+    }
+    double orig[3] = {(double)(N[0] - 1) / 2.0, (double)(N[1] - 1) / 2.0, (double)(N[2] - 1) / 2.0};
+    const double *d = p->d;
+    auto V = [=](double x, double y, double z) -> double {
+        if (!vbox) return 0.0;
+        long i = lround(x / d[0] + orig[0]) - lo[0];
+        long j = lround(y / d[1] + orig[1]) - lo[1];
+        long k = lround(z / d[2] + orig[2]) - lo[2];
+        if (i < 0 || j < 0 || k < 0 || i >= n[0] || j >= n[1] || k >= n[2]) return 0.0;
+        return vbox[(i * n[1] + j) * n[2] + k];
+    };
+    Oracle *O = build(p, V, *rc);
+    return O;
+}
+
+void *orc_create_hook(const pstd_params *p, pstd_potential_fn fn, int *rc) {
+    auto V = [=](double x, double y, double z) -> double { return fn ? fn(x, y, z) : 0.0; };
+    return build(p, V, *rc);
+}
+
+void orc_destroy(void *h) { delete (Oracle *)h; }
+
+int orc_update(void *h, long n) {
+    Oracle &O = *(Oracle *)h;
+    for (long i = 0; i < n; ++i) update(O);
+    return 0;
+}
+int orc_accumulate(void *h) {
+    accumulate(*(Oracle *)h);
+    return 0;
+}
+int orc_update_accumulate(void *h, long n) {
+    Oracle &O = *(Oracle *)h;
+    for (long i = 0; i < n; ++i) { update(O); accumulate(O); }
+    return 0;
+}
+long orc_steps_per_period(void *h) { return (long)(TWOPI / ((Oracle *)h)->p.dtau + 0.5); }  // :1733-1736
+long orc_time_index(void *h) { return ((Oracle *)h)->nTau; }
+long orc_surface_count(void *h) { return ((Oracle *)h)->nSur; }
+void orc_set_time_index(void *h, long t) { ((Oracle *)h)->nTau = t; }
+
+/* global geometry: N[3] abc1[3] abc2[3] k1a k1b k2a k2b vr1 vr2 inj  (33 longs) */
+void orc_global_info(void *h, long *out) {
+    Oracle &O = *(Oracle *)h;
+    for (int a = 0; a < 3; ++a) {
+        out[a] = O.N[a]; out[3 + a] = O.abc1[a]; out[6 + a] = O.abc2[a];
+        out[9 + a] = O.k1a[a]; out[12 + a] = O.k1b[a]; out[15 + a] = O.k2a[a]; out[18 + a] = O.k2b[a];
+        out[21 + a] = O.vr1[a]; out[24 + a] = O.vr2[a]; out[27 + a] = O.inj[a];
+    }
+    out[30] = (long)O.ranks.size();
+}
+long orc_num_ranks(void *h) { return (long)((Oracle *)h)->ranks.size(); }
+
+/* per rank: c[3] f1[3] f2[3] c1[3] c2[3] flgV v1[3] v2[3] flgKsi[6]
+ * slab[6][6] flgVrtl[6]   (3+3+3+3+3+1+3+3+6+36+6 = 70 longs) */
+void orc_rank_info(void *h, long r, long *out) {
+    Rank &R = ((Oracle *)h)->ranks[r];
+    long *o = out;
+    for (int a = 0; a < 3; ++a) *o++ = R.c[a];
+    for (int a = 0; a < 3; ++a) *o++ = R.f1[a];
+    for (int a = 0; a < 3; ++a) *o++ = R.f2[a];
+    for (int a = 0; a < 3; ++a) *o++ = R.c1[a];
+    for (int a = 0; a < 3; ++a) *o++ = R.c2[a];
+    *o++ = R.flgV;
+    for (int a = 0; a < 3; ++a) *o++ = R.flgV ? R.v1[a] : -1;
+    for (int a = 0; a < 3; ++a) *o++ = R.flgV ? R.v2[a] : -1;
+    for (int s = 0; s < 6; ++s) *o++ = R.flgKsi[s];
+    for (int s = 0; s < 6; ++s) {
+        const Slab &S = R.slab[s];
+        if (R.flgKsi[s]) { *o++ = S.xa; *o++ = S.xb; *o++ = S.ya; *o++ = S.yb; *o++ = S.za; *o++ = S.zb; }
+        else for (int q = 0; q < 6; ++q) *o++ = -1;
+    }
+    for (int f = 0; f < 6; ++f) *o++ = R.flgVrtl[f];
+}
+
+/* local box of rank r: which 0 = m_Psi, 1 = m_Psi0; [Lx][Ly][2Lz] */
+void orc_get_local(void *h, long r, int which, double *out) {
+    Rank &R = ((Oracle *)h)->ranks[r];
+    const std::vector<double> &src = which == 0 ? R.psi : R.psi0;
+    std::memcpy(out, src.data(), sizeof(double) * src.size());
+}
+
+/* global psi from the cores: level 0 = m_Psi, 1 = m_Psi0 */
+void orc_get_psi(void *h, int level, double *out) {
+    Oracle &O = *(Oracle *)h;
+    const long Ny = O.N[1], Nz = O.N[2];
+    for (Rank &R : O.ranks) {
+        const std::vector<double> &src = level == 0 ? R.psi : R.psi0;
+        for (long i = R.c1[0]; i <= R.c2[0]; ++i)
+            for (long j = R.c1[1]; j <= R.c2[1]; ++j)
+                for (long k = R.c1[2]; k <= R.c2[2]; ++k)
+                    for (int e = 0; e < 2; ++e)
+                        out[((i * Ny + j) * Nz + k) * 2 + e] =
+                            src[(i - R.f1[0]) * R.nyz + (j - R.f1[1]) * R.nz + (k - R.f1[2]) * 2 + e];
+    }
+}
+
+/* write global psi into the cores; level 0 then runs the exchange */
+void orc_set_psi(void *h, int level, const double *in) {
+    Oracle &O = *(Oracle *)h;
+    const long Ny = O.N[1], Nz = O.N[2];
+    for (Rank &R : O.ranks) {
+        std::vector<double> &dst = level == 0 ? R.psi : R.psi0;
+        std::fill(dst.begin(), dst.end(), 0.0);
+        for (long i = R.c1[0]; i <= R.c2[0]; ++i)
+            for (long j = R.c1[1]; j <= R.c2[1]; ++j)
+                for (long k = R.c1[2]; k <= R.c2[2]; ++k)
+                    for (int e = 0; e < 2; ++e)
+                        dst[(i - R.f1[0]) * R.nyz + (j - R.f1[1]) * R.nz + (k - R.f1[2]) * 2 + e] =
+                            in[((i * Ny + j) * Nz + k) * 2 + e];
+    }
+    if (level == 0) exchange(O);
+}
+
+/* sum of the six Inc slabs on the global grid (complex), zero elsewhere */
+void orc_get_source(void *h, double *out) {
+    Oracle &O = *(Oracle *)h;
+    const long Nx = O.N[0], Ny = O.N[1], Nz = O.N[2];
+    std::memset(out, 0, sizeof(double) * Nx * Ny * Nz * 2);
+    for (Rank &R : O.ranks)
+        for (int s = 0; s < 6; ++s) {
+            if (!R.flgKsi[s]) continue;
+            const Slab &S = R.slab[s];
+            if (S.nx <= 0 || S.ny <= 0 || S.nz <= 0) continue;
+            long nz = S.nz, nyz = S.ny * nz;
+            for (long i = S.xa; i <= S.xb; ++i)
+                for (long j = S.ya; j <= S.yb; ++j)
+                    for (long k = S.za; k <= S.zb; ++k)
+                        for (int e = 0; e < 2; ++e)
+                            out[((i * Ny + j) * Nz + k) * 2 + e] +=
+                                R.inc[s][(i - S.xa) * nyz + (j - S.ya) * nz + (k - S.za) * 2 + e];
+        }
+}
+
+/* ABC factor g1(i) g2(j) g3(k) along axis a, global index */
+void orc_get_g(void *h, int a, double *out) {
+    Oracle &O = *(Oracle *)h;
+    for (Rank &R : O.ranks)
+        for (long i = R.c1[a]; i <= R.c2[a]; ++i) out[i] = R.g[a][i - R.c1[a]];
+}
+
+/* V~ = V 2 dtau / E0 on the global grid, zero outside the potential box */
+void orc_get_vtilde(void *h, double *out) {
+    Oracle &O = *(Oracle *)h;
+    const long Ny = O.N[1], Nz = O.N[2];
+    std::memset(out, 0, sizeof(double) * O.N[0] * Ny * Nz);
+    for (Rank &R : O.ranks) {
+        if (!R.flgV) continue;
+        for (long i = R.v1[0]; i <= R.v2[0]; ++i)
+            for (long j = R.v1[1]; j <= R.v2[1]; ++j)
+                for (long k = R.v1[2]; k <= R.v2[2]; ++k)
+                    out[(i * Ny + j) * Nz + k] =
+                        R.V[((i - R.v1[0]) * R.nV[1] + (j - R.v1[1])) * R.nV[2] + (k - R.v1[2])];
+    }
+}
+
+int orc_get_surfaces(void *h, int deriv, double *out, long nvals) {
+    std::vector<double> body;
+    gather_surfaces(*(Oracle *)h, deriv, body, true);
+    if ((long)body.size() != nvals) return fail(PSTD_EINVAL, "surface size mismatch");
+    std::memcpy(out, body.data(), sizeof(double) * body.size());
+    return 0;
+}
+
+/* SaveVirtualSurfaces, PSTD_QScat.cpp:2390-2404 */
+int orc_save_virtual_surfaces(void *h, const char *prefix) {
+    Oracle &O = *(Oracle *)h;
+    if (!O.nSur) {
+        std::fprintf(stderr, "Warning: Virtual Surface terms empty.  No file to be saved.\n");
+        return 0;
+    }
+    long n0 = O.nTau - O.nSur;
+    int rc = write_file(O, std::string(prefix) + "_PS_" + std::to_string(n0) + ".dat", PSTD_DERIV_PS);
+    if (rc) return rc;
+    rc = write_file(O, std::string(prefix) + "_FD_" + std::to_string(n0) + ".dat", PSTD_DERIV_FD);
+    if (rc) return rc;
+    reset_surfaces(O);
+    return 0;
+}
+
+void orc_reset_surface_terms(void *h) { reset_surfaces(*(Oracle *)h); }
+
+}  // extern "C"

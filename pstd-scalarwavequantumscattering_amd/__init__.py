@@ -1,0 +1,18 @@
+"""MI355X-native PSTD_QScat time-stepping core (host-side Python mirror).
+
+``PSTDQScat`` mirrors the public interface of the reference's CPSTD_QScat
+(PSTD_QScat.h:203-223): ``Init`` / ``Update`` /
+``IterationsToAccumulateSurfaceTerms`` / ``AccumulateSurfaceTerms`` /
+``SaveVirtualSurfaces``, over the C ABI of ``libpstd_amd.so``
+(include/pstd.h).  All arithmetic runs in the HIP kernels of that library;
+this module only marshals arguments.  If the library is missing the import
+of the solver fails loudly: there is no CPU fallback.
+"""
+from .params import (  # noqa: F401
+    MAX_SEG, NWEIGHT, PSTD_DERIV_FD, PSTD_DERIV_PS, PSTD_EINPUT, PSTD_EINVAL,
+    PSTD_EIO, PSTD_ENODEV, PSTD_ENOMEM, PSTD_ETOPO, PSTD_FP32, PSTD_FP64,
+    PSTD_OK, PstdGeometry, PstdParams, global_size, make_params, surface_body_len,
+)
+from .solver import (  # noqa: F401
+    LIB_PATH, PSTDError, PSTDQScat, compute_geometry, library, load_input,
+)

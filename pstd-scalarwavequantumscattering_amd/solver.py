@@ -1,0 +1,196 @@
+"""ctypes binding of libpstd_amd.so and the CPSTD_QScat-shaped wrapper.
+
+Method names follow the reference class (PSTD_QScat.h:203-223); argument
+meaning and error behaviour follow it too, with the reference's
+``runtime_error`` turned into ``PSTDError`` carrying the same error code
+(PSTD_QScat.cpp:2557-2564, qscat.cpp:75-79).
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .params import (PSTD_DERIV_FD, PSTD_DERIV_PS, PSTD_FP64, PstdGeometry,
+                     PstdParams, surface_body_len)
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libpstd_amd.so"
+
+_lib = None
+
+EXPORTS = [
+    "pstd_load_input", "pstd_compute_geometry", "pstd_create", "pstd_create_vbox",
+    "pstd_destroy", "pstd_update", "pstd_steps_per_period",
+    "pstd_accumulate_surface_terms", "pstd_update_accumulate", "pstd_surface_count",
+    "pstd_time_index", "pstd_get_surfaces", "pstd_save_virtual_surfaces",
+    "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
+    "pstd_synchronize", "pstd_last_error",
+]
+
+
+class PSTDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"Error {code}: {msg}")
+        self.code = code
+
+
+def library():
+    """Load libpstd_amd.so (built by __graft_entry__.build()).  Raises if the
+    library has not been built: the solver has no other implementation."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        vp, i, l, sz = C.c_void_p, C.c_int, C.c_long, C.c_size_t
+        dp = C.POINTER(C.c_double)
+        L.pstd_load_input.restype = i
+        L.pstd_load_input.argtypes = [C.c_char_p, C.POINTER(PstdParams)]
+        L.pstd_compute_geometry.restype = i
+        L.pstd_compute_geometry.argtypes = [C.POINTER(PstdParams), C.POINTER(PstdGeometry)]
+        L.pstd_create.restype = i
+        L.pstd_create.argtypes = [C.POINTER(PstdParams), vp, C.POINTER(vp)]
+        L.pstd_create_vbox.restype = i
+        L.pstd_create_vbox.argtypes = [C.POINTER(PstdParams), dp, sz, C.POINTER(vp)]
+        L.pstd_destroy.restype = None
+        L.pstd_destroy.argtypes = [vp]
+        for name in ("pstd_update", "pstd_update_accumulate"):
+            getattr(L, name).restype = i
+            getattr(L, name).argtypes = [vp, l]
+        for name in ("pstd_steps_per_period", "pstd_surface_count", "pstd_time_index"):
+            getattr(L, name).restype = l
+            getattr(L, name).argtypes = [vp]
+        for name in ("pstd_accumulate_surface_terms", "pstd_reset_surface_terms", "pstd_synchronize"):
+            getattr(L, name).restype = i
+            getattr(L, name).argtypes = [vp]
+        L.pstd_get_surfaces.restype = i
+        L.pstd_get_surfaces.argtypes = [vp, i, dp, sz]
+        L.pstd_save_virtual_surfaces.restype = i
+        L.pstd_save_virtual_surfaces.argtypes = [vp, C.c_char_p]
+        L.pstd_get_psi.restype = i
+        L.pstd_get_psi.argtypes = [vp, i, dp, sz]
+        L.pstd_set_psi.restype = i
+        L.pstd_set_psi.argtypes = [vp, i, dp, sz]
+        L.pstd_set_time_index.restype = i
+        L.pstd_set_time_index.argtypes = [vp, l]
+        L.pstd_last_error.restype = C.c_char_p
+        L.pstd_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc:
+        raise PSTDError(rc, library().pstd_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def load_input(path):
+    """qscat.cpp:214-297 load_data: parse a reference input file."""
+    p = PstdParams()
+    _check(library().pstd_load_input(str(path).encode(), C.byref(p)))
+    return p
+
+
+def compute_geometry(p):
+    """CPSTD_QScat::DomainDecomposition tables (PSTD_QScat.cpp:270-396)."""
+    g = PstdGeometry()
+    _check(library().pstd_compute_geometry(C.byref(p), C.byref(g)))
+    return g
+
+
+class PSTDQScat:
+    """The CPSTD_QScat interface over the MI355X library.
+
+    ``PSTDQScat(params, vbox)`` is ``Init`` (PSTD_QScat.cpp:108-236) with the
+    potential given sampled on the potential box (``vbox``, E0 units, x
+    slowest); ``vbox=None`` means V = 0.  ``hook`` may instead be a ctypes
+    ``CFUNCTYPE(c_double, c_double, c_double, c_double)`` potential hook
+    (run_environment.h:82)."""
+
+    def __init__(self, params, vbox=None, hook=None):
+        self.params = params
+        h = C.c_void_p()
+        L = library()
+        if hook is not None:
+            self._hook = hook
+            _check(L.pstd_create(C.byref(params), C.cast(hook, C.c_void_p), C.byref(h)))
+        elif vbox is None:
+            _check(L.pstd_create(C.byref(params), None, C.byref(h)))
+        else:
+            vb = np.ascontiguousarray(vbox, dtype=np.float64)
+            _check(L.pstd_create_vbox(C.byref(params), _dp(vb), vb.size, C.byref(h)))
+        self.h = h
+        g = compute_geometry(params)
+        self.geometry = g
+        self.N = tuple(g.N)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            library().pstd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- reference-named API ---------------------------------------------
+    def Update(self, nsteps=1):
+        """PSTD_QScat.cpp:1233-1446 (nsteps times)."""
+        _check(library().pstd_update(self.h, int(nsteps)))
+
+    def IterationsToAccumulateSurfaceTerms(self):
+        """PSTD_QScat.cpp:1733-1736."""
+        return library().pstd_steps_per_period(self.h)
+
+    def AccumulateSurfaceTerms(self):
+        """PSTD_QScat.cpp:1738-2052."""
+        _check(library().pstd_accumulate_surface_terms(self.h))
+
+    def SaveVirtualSurfaces(self, prefix):
+        """PSTD_QScat.cpp:2390-2404."""
+        _check(library().pstd_save_virtual_surfaces(self.h, str(prefix).encode()))
+
+    # --- extras ------------------------------------------------------------
+    def update_accumulate(self, nsteps):
+        _check(library().pstd_update_accumulate(self.h, int(nsteps)))
+
+    def synchronize(self):
+        _check(library().pstd_synchronize(self.h))
+
+    def time_index(self):
+        return library().pstd_time_index(self.h)
+
+    def set_time_index(self, t):
+        _check(library().pstd_set_time_index(self.h, int(t)))
+
+    def surface_count(self):
+        return library().pstd_surface_count(self.h)
+
+    def reset_surface_terms(self):
+        _check(library().pstd_reset_surface_terms(self.h))
+
+    def get_surfaces(self, deriv=PSTD_DERIV_PS):
+        n = surface_body_len(self.N)
+        out = np.zeros(n)
+        _check(library().pstd_get_surfaces(self.h, int(deriv), _dp(out), n))
+        return out
+
+    def get_psi(self, level=0):
+        n = int(np.prod(self.N)) * 2
+        out = np.zeros(self.N + (2,))
+        _check(library().pstd_get_psi(self.h, int(level), _dp(out), n))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def set_psi(self, level, psi):
+        a = np.ascontiguousarray(np.stack([psi.real, psi.imag], axis=-1), dtype=np.float64)
+        _check(library().pstd_set_psi(self.h, int(level), _dp(a), a.size))
+
+
+__all__ = ["PSTDQScat", "PSTDError", "load_input", "compute_geometry", "library",
+           "LIB_PATH", "EXPORTS", "PSTD_DERIV_PS", "PSTD_DERIV_FD", "PSTD_FP64"]

@@ -1,0 +1,224 @@
+"""Workload builders: solver parameters and sampled potentials.
+
+Physical inputs use hbar = m = 1.  They map to the reference's scaled units
+(length in 1/k, time in omega t; PSTD_QScat.cpp:1127-1136, potential.cpp)
+as
+
+    dx_ref = k dx,     E0 = k^2 / 2,     V_hook(x') = V_phys(x' / k),
+    dtau   = 0.5 * dx_ref^2 / (3 pi^2)   (half the leapfrog stability limit
+                                           of three spectral axes).
+
+Random inputs use SplitMix64 (seed -> uniform doubles in [0, 1)) and
+Box-Muller normals, so every potential is bit-reproducible on any host.
+Potentials are built in FP64 on the host and handed to the solver (and to
+the CPU oracle) already sampled on the potential box.
+"""
+import math
+
+import numpy as np
+
+from .params import PSTD_FP32, PSTD_FP64, global_size, make_params
+
+# --------------------------------------------------------------------------
+# deterministic random numbers
+# --------------------------------------------------------------------------
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014); uniform() uses the top 53 bits."""
+
+    def __init__(self, seed):
+        self.state = int(seed) & _MASK
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, n=None):
+        if n is None:
+            return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+        return np.array([(self.next_u64() >> 11) * (1.0 / (1 << 53)) for _ in range(n)])
+
+    def uniform_array(self, n):
+        """Vectorised stream of n uniforms, identical to n calls of uniform()."""
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            self.state = int(z[-1]) if n else self.state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+    def normal_array(self, n):
+        """Box-Muller: pairs (u1, u2) -> sqrt(-2 ln(1-u1)) cos/sin(2 pi u2)."""
+        m = (n + 1) // 2
+        u = self.uniform_array(2 * m).reshape(m, 2)
+        r = np.sqrt(-2.0 * np.log1p(-u[:, 0]))
+        t = 2.0 * math.pi * u[:, 1]
+        return np.stack([r * np.cos(t), r * np.sin(t)], axis=1).reshape(-1)[:n]
+
+
+def random_direction(rng):
+    """Uniform direction on the sphere -> (theta, phi)."""
+    u1, u2 = rng.uniform(), rng.uniform()
+    return math.acos(1.0 - 2.0 * u1), 2.0 * math.pi * u2
+
+
+# --------------------------------------------------------------------------
+# geometry helpers
+# --------------------------------------------------------------------------
+def stable_dtau(dx_ref, fraction=0.5):
+    return fraction * dx_ref * dx_ref / (3.0 * math.pi ** 2)
+
+
+def vbox_range(p):
+    """Global potential box (ksi1b, ksi2a) exclusive, per axis
+    (PSTD_QScat.cpp:368-387 united over ranks)."""
+    N = global_size(p)
+    lo, hi = [], []
+    for a in range(3):
+        abc1, abc2 = p.n_abc - 1, N[a] - p.n_abc
+        k1b = abc1 + p.n_sca + 1 + p.n_ksi - 1
+        k2a = abc2 - p.n_sca - 1 - p.n_ksi + 1
+        lo.append(k1b + 1)
+        hi.append(k2a - 1)
+    return tuple(lo), tuple(hi)
+
+
+def vbox_coords(p):
+    """Scaled, centre-relative coordinates of the potential box points
+    (PSTD_QScat.cpp:1126-1135)."""
+    N = global_size(p)
+    lo, hi = vbox_range(p)
+    ax = [(np.arange(lo[a], hi[a] + 1) - (N[a] - 1) / 2.0) * p.d[a] for a in range(3)]
+    return ax
+
+
+def sample_hook(p, fn):
+    """Sample a scalar hook V(x, y, z) (numpy-vectorised) on the box."""
+    x, y, z = vbox_coords(p)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    return np.ascontiguousarray(fn(X, Y, Z), dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# potentials (physical units; k converts to scaled coordinates)
+# --------------------------------------------------------------------------
+def square_well(p, k, R, V0):
+    """Spherical well of radius R and depth V0 at the grid centre."""
+    return sample_hook(p, lambda X, Y, Z: np.where((X / k) ** 2 + (Y / k) ** 2 + (Z / k) ** 2 <= R * R, V0, 0.0))
+
+
+def example_barrier(p):
+    """potential.cpp:36-42: barrier of height 0.05 E (E = 0.327216805 in the
+    example input) and radius 2 pi, in scaled units."""
+    V0 = 0.327216805 * 0.05
+    R2 = (2.0 * math.pi) ** 2
+    return sample_hook(p, lambda X, Y, Z: np.where(X * X + Y * Y + Z * Z <= R2, V0, 0.0))
+
+
+def gaussian_random_field(p, k, dx_phys, sigma, corr_len, cutoff, seed):
+    """White noise (seeded) convolved with a Gaussian exp(-r^2 / (2 s^2)),
+    s = corr_len / sqrt(2), scaled to standard deviation sigma, times a hard
+    spherical cutoff of radius `cutoff` (physical units)."""
+    lo, hi = vbox_range(p)
+    shape = tuple(hi[a] - lo[a] + 1 for a in range(3))
+    rng = SplitMix64(seed)
+    noise = rng.normal_array(int(np.prod(shape))).reshape(shape)
+    s = corr_len / math.sqrt(2.0)
+    freqs = [np.fft.fftfreq(shape[a], d=dx_phys) for a in range(3)]
+    KX, KY, KZ = np.meshgrid(*freqs, indexing="ij")
+    # Fourier transform of exp(-r^2/(2 s^2)) up to a constant
+    G = np.exp(-2.0 * math.pi ** 2 * s * s * (KX ** 2 + KY ** 2 + KZ ** 2))
+    field = np.real(np.fft.ifftn(np.fft.fftn(noise) * G))
+    field *= sigma / field.std()
+    x, y, z = vbox_coords(p)
+    X, Y, Z = np.meshgrid(x / k, y / k, z / k, indexing="ij")
+    field[X * X + Y * Y + Z * Z > cutoff * cutoff] = 0.0
+    return np.ascontiguousarray(field)
+
+
+def gaussian_wells(p, k, n, ball, wmin, wmax, hmin, hmax, seed):
+    """Sum of n Gaussians: centres uniform in a ball, widths and heights
+    uniform (physical units)."""
+    rng = SplitMix64(seed)
+    x, y, z = vbox_coords(p)
+    X, Y, Z = np.meshgrid(x / k, y / k, z / k, indexing="ij")
+    V = np.zeros(X.shape)
+    for _ in range(n):
+        while True:
+            c = np.array([2 * rng.uniform() - 1 for _ in range(3)])
+            if c @ c <= 1.0:
+                break
+        c *= ball
+        w = wmin + (wmax - wmin) * rng.uniform()
+        h = hmin + (hmax - hmin) * rng.uniform()
+        V += h * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * w * w))
+    return np.ascontiguousarray(V)
+
+
+# --------------------------------------------------------------------------
+# named workloads
+# --------------------------------------------------------------------------
+def config_well96(steps=3000, precision=PSTD_FP64):
+    """96^3, dx = 0.25, 2x1x1 subdomains, L = (64, 96, 96), h = 16,
+    12-cell absorber, TF/SF inset 8 (n_sca = 7), square well R = 3,
+    V0 = -2, k = 2 along +z, 3000 steps."""
+    k, dx = 2.0, 0.25
+    dxr = k * dx
+    p = make_params((2, 1, 1), stable_dtau(dxr), (dxr,) * 3, (64, 96, 96), 12, 7, 12, 16,
+                    0.0, 0.0, k * k / 2, 5.0, 4.0 / 12, steps, "well96", precision)
+    return p, square_well(p, k, 3.0, -2.0)
+
+
+def config_grf224(steps=6000, precision=PSTD_FP64, seed=2):
+    """224^3 (L = 80, h = 16, 4x4x4 subdomains: interior cores 48, edge
+    cores 64), dx = 0.2, k = 3 along a seeded direction, Gaussian-correlated
+    random potential (corr. length 1, sigma 0.5, cutoff r = 12), 14-cell
+    absorber, 6000 steps."""
+    k, dx = 3.0, 0.2
+    dxr = k * dx
+    rng = SplitMix64(seed)
+    th, ph = random_direction(rng)
+    p = make_params((4, 4, 4), stable_dtau(dxr), (dxr,) * 3, (80, 80, 80), 14, 7, 12, 16,
+                    th, ph, k * k / 2, 5.0, 4.0 / 14, steps, "grf224", precision)
+    return p, gaussian_random_field(p, k, dx, 0.5, 1.0, 12.0, seed)
+
+
+def config_wells320(steps=8000, precision=PSTD_FP32, seed=3):
+    """320^3 (L = 80, h = 16, 6x6x6 subdomains), dx = 0.2, k = 2.5 along a
+    seeded direction, 20 seeded Gaussian wells/barriers in a ball of radius
+    15, 14-cell absorber, 8000 steps, FP32 with FP64 phase."""
+    k, dx = 2.5, 0.2
+    dxr = k * dx
+    rng = SplitMix64(seed)
+    th, ph = random_direction(rng)
+    p = make_params((6, 6, 6), stable_dtau(dxr), (dxr,) * 3, (80, 80, 80), 14, 7, 12, 16,
+                    th, ph, k * k / 2, 5.0, 4.0 / 14, steps, "wells320", precision)
+    return p, gaussian_wells(p, k, 20, 15.0, 0.5, 2.0, -3.0, 3.0, seed + 1000)
+
+
+def config_example():
+    """InputExample.txt: 1x2x1, dtau = pi/1000, dx = 2 pi/20, L = 288 160 288,
+    40 41 12 15, theta = phi = pi/2, E0 = 0.327216805, U0 = 5, alpha = 0.1,
+    50000 steps, prefix barrier_50K_U5; potential.cpp's barrier."""
+    pi = math.pi
+    p = make_params((1, 2, 1), pi / 1000, (2 * pi / 20,) * 3, (288, 160, 288), 40, 41, 12, 15,
+                    pi / 2, pi / 2, 0.327216805, 5.0, 0.1, 50000, "barrier_50K_U5")
+    return p, example_barrier(p)
+
+
+def config_tiny(nprocs=(2, 2, 2), L=(32, 32, 32), h=10, n_abc=4, n_sca=3, n_ksi=4,
+                k=2.0, dx=0.25, theta=0.3, phi=0.7, well=(1.0, -2.0), precision=PSTD_FP64,
+                U0=5.0):
+    """Small grids for parity tests (N = P (L - 2h) + 2h per axis)."""
+    dxr = k * dx
+    p = make_params(nprocs, stable_dtau(dxr), (dxr,) * 3, L, n_abc, n_sca, n_ksi, h,
+                    theta, phi, k * k / 2, U0, 4.0 / max(n_abc, 1), 100, "tiny", precision)
+    V = square_well(p, k, well[0], well[1]) if well else None
+    return p, V
