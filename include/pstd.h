@@ -188,6 +188,27 @@ int pstd_set_time_index(pstd_solver *s, long ntau);
 /* Block until all queued work of the solver has finished. */
 int pstd_synchronize(pstd_solver *s);
 
+/* ---- measurement and inspection ------------------------------------- */
+
+/* Run nsteps Updates followed by naccum AccumulateSurfaceTerms calls with
+ * HIP events around every kernel launch on the solver's stream; write the
+ * average duration in milliseconds per launch of each kernel to ms_out:
+ * [0] x pass, [1] y pass, [2] z pass (+ potential, TF/SF, ABC),
+ * [3] surface capture (one launch per axis).  The state advances as with
+ * pstd_update / pstd_update_accumulate.  No reference counterpart (the
+ * reference times with MPI_Wtime, qscat.cpp:165-172). */
+int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]);
+
+/* The dense real line operator the kernels apply for axis a and segment r:
+ * kind 0: alpha * D2[core rows][0..L) * W (ncore x L, row-major), the
+ *         windowed local-Fourier second derivative of UpdateX/Y/Z
+ *         (PSTD_QScat.cpp:1448-1537) with alpha = 2 sin(dtau);
+ * kind 1: the surface rows of face 2a (r ignored): PS weights then FD
+ *         weights, 2 x L (PSTD_QScat.cpp:1804-1820);
+ * kind 2: as kind 1 for face 2a+1.
+ * Host only; no device needed.  nvals must match exactly. */
+int pstd_line_operator(const pstd_params *p, int axis, int seg, int kind, double *out, size_t nvals);
+
 /* Last error message of this thread ("" if none). */
 const char *pstd_last_error(void);
 

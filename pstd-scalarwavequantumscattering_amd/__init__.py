@@ -14,5 +14,5 @@ from .params import (  # noqa: F401
     PSTD_OK, PstdGeometry, PstdParams, global_size, make_params, surface_body_len,
 )
 from .solver import (  # noqa: F401
-    LIB_PATH, PSTDError, PSTDQScat, compute_geometry, library, load_input,
+    LIB_PATH, PSTDError, PSTDQScat, compute_geometry, library, line_operator, load_input,
 )

@@ -1,0 +1,417 @@
+// Geometry, coefficient tables and input parsing (host, FP64 / long double).
+#include "pstd_host.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace pstd {
+
+namespace {
+const double TINY = 1.0e-10;                                   // run_environment.h:63
+const double TWOPI = 6.283185307179586476925286766559005768394;
+const double FOURPI = 12.56637061435917295385057353311801153679;
+const long double PI_L = 3.141592653589793238462643383279502884197L;
+// SMOOTH4 coefficients (utility.cpp:39-45)
+const double A0 = 1.0, A1 = -4.0 / 3.0, A2 = 1.0 / 3.0;
+// FD8 coefficients (PSTD_QScat.cpp:43-46)
+const double FDC[4] = {4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0};
+}  // namespace
+
+double ksi(double x) {
+    if (x < TINY) return 0.0;
+    if (x > 1.0 - TINY) return 1.0;
+    return A0 * x + (A1 / TWOPI) * std::sin(TWOPI * x) + (A2 / FOURPI) * std::sin(FOURPI * x);
+}
+double dksi(double x) {
+    if (x < TINY || x > 1.0 - TINY) return 0.0;
+    return A0 + A1 * std::cos(TWOPI * x) + A2 * std::cos(FOURPI * x);
+}
+double d2ksi(double x) {
+    if (x < TINY || x > 1.0 - TINY) return 0.0;
+    return -TWOPI * (A1 * std::sin(TWOPI * x) + 2.0 * A2 * std::sin(FOURPI * x));
+}
+
+static void build_axis(Axis &a, long P, long L, long h, double d) {
+    a.P = P; a.L = L; a.h = h; a.d = d;
+    a.N = P * (L - 2 * h) + 2 * h;
+    a.fft1.assign(P, 0); a.core1.assign(P, 0); a.core2.assign(P, 0);
+    for (long r = 0; r < P; ++r) {
+        if (P == 1) {
+            a.fft1[r] = 0; a.core1[r] = 0; a.core2[r] = L - 1;
+        } else if (r == 0) {
+            a.fft1[r] = 0; a.core1[r] = 0; a.core2[r] = L - 1 - h;
+        } else {
+            a.fft1[r] = r * (L - 2 * h);
+            a.core1[r] = a.fft1[r] + h;
+            a.core2[r] = (r == P - 1) ? a.fft1[r] + L - 1 : a.fft1[r] + L - 1 - h;
+        }
+    }
+}
+
+int build_geometry(const pstd_params &p, Geometry &g, std::string &err) {
+    std::ostringstream m;
+    for (int a = 0; a < 3; ++a) {
+        if (p.nprocs[a] < 1 || p.nprocs[a] > PSTD_MAX_SEG) {
+            m << "subdomain count on axis " << a << " must be 1.." << PSTD_MAX_SEG;
+            err = m.str();
+            return PSTD_ETOPO;
+        }
+        if (p.L[a] < 4 || !(p.d[a] > 0.0)) {
+            m << "axis " << a << ": L must be >= 4 and spacing > 0";
+            err = m.str();
+            return PSTD_EINVAL;
+        }
+        if (p.nprocs[a] > 1) {
+            // windows on planes 1..8 must stay inside the exchanged halo
+            if (p.n_halo < PSTD_NWEIGHT + 1) {
+                err = "n_halo must be >= 9 when an axis has more than one subdomain";
+                return PSTD_EINVAL;
+            }
+            long edge = p.L[a] - p.n_halo, inner = p.L[a] - 2 * p.n_halo;
+            if (edge < p.n_halo - 1 || (p.nprocs[a] > 2 && inner < p.n_halo - 1)) {
+                m << "axis " << a << ": core shorter than the h-1 exchanged planes";
+                err = m.str();
+                return PSTD_EINVAL;
+            }
+        }
+        build_axis(g.ax[a], p.nprocs[a], p.L[a], p.n_halo, p.d[a]);
+    }
+    if (p.n_ksi < 2 || !(p.dtau > 0.0) || p.E0 == 0.0 || p.n_sca < 0 || p.n_abc < 1) {
+        err = "need n_ksi >= 2, n_abc >= 1, n_sca >= 0, dtau > 0, E0 != 0";
+        return PSTD_EINVAL;
+    }
+    if (p.precision != PSTD_FP64 && p.precision != PSTD_FP32) {
+        err = "precision must be 8 (FP64) or 4 (FP32)";
+        return PSTD_EINVAL;
+    }
+    for (int a = 0; a < 3; ++a) {
+        long N = g.ax[a].N;
+        g.abc1[a] = p.n_abc - 1;
+        g.abc2[a] = N - p.n_abc;
+        g.ksi1a[a] = g.abc1[a] + p.n_sca + 1;
+        g.ksi1b[a] = g.ksi1a[a] + p.n_ksi - 1;
+        g.ksi2b[a] = g.abc2[a] - p.n_sca - 1;
+        g.ksi2a[a] = g.ksi2b[a] - p.n_ksi + 1;
+        g.vrtl1[a] = g.abc1[a] + (p.n_sca + 1) / 2;
+        g.vrtl2[a] = g.abc2[a] - (p.n_sca + 1) / 2;
+        g.vbox1[a] = g.ksi1b[a] + 1;
+        g.vbox2[a] = g.ksi2a[a] - 1;
+        g.orig[a] = (double)(N - 1) / 2.0;
+        if (g.ksi1a[a] < 0 || g.ksi2b[a] >= N || g.ksi2b[a] < g.ksi1a[a]) {
+            m << "axis " << a << ": TF/SF box does not fit the grid";
+            err = m.str();
+            return PSTD_EINVAL;
+        }
+        if (g.vrtl1[a] < 0 || g.vrtl2[a] >= N) {
+            m << "axis " << a << ": virtual surface outside the grid";
+            err = m.str();
+            return PSTD_EINVAL;
+        }
+    }
+    const double ct = std::cos(p.theta), st = std::sin(p.theta);
+    const double cp = std::cos(p.phi), sp = std::sin(p.phi);
+    g.khat[0] = st * cp;
+    g.khat[1] = st * sp;
+    g.khat[2] = ct;
+    // injection corner from the octant of (theta, phi), PSTD_QScat.cpp:625-653
+    g.inj[0] = (cp >= 0.0) ? g.ksi1a[0] : g.ksi2b[0];
+    g.inj[1] = (sp >= 0.0) ? g.ksi1a[1] : g.ksi2b[1];
+    g.inj[2] = (ct >= 0.0) ? g.ksi1a[2] : g.ksi2b[2];
+    // the surface stencils must stay inside the owning segment's local line
+    for (int f = 0; f < 6; ++f) {
+        SurfaceOp op;
+        int rc = build_surface_op(g, f, op, err);
+        if (rc) return rc;
+    }
+    return PSTD_OK;
+}
+
+void export_geometry(const Geometry &g, pstd_geometry *o) {
+    std::memset(o, 0, sizeof(*o));
+    for (int a = 0; a < 3; ++a) {
+        o->N[a] = g.ax[a].N;
+        o->abc1[a] = g.abc1[a]; o->abc2[a] = g.abc2[a];
+        o->ksi1a[a] = g.ksi1a[a]; o->ksi1b[a] = g.ksi1b[a];
+        o->ksi2a[a] = g.ksi2a[a]; o->ksi2b[a] = g.ksi2b[a];
+        o->vrtl1[a] = g.vrtl1[a]; o->vrtl2[a] = g.vrtl2[a];
+        o->vbox1[a] = g.vbox1[a]; o->vbox2[a] = g.vbox2[a];
+        o->inj[a] = g.inj[a];
+        o->orig[a] = g.orig[a];
+        o->n_seg[a] = g.ax[a].P;
+        for (long r = 0; r < g.ax[a].P; ++r) {
+            o->fft1[a][r] = g.ax[a].fft1[r];
+            o->fft2[a][r] = g.ax[a].fft1[r] + g.ax[a].L - 1;
+            o->core1[a][r] = g.ax[a].core1[r];
+            o->core2[a][r] = g.ax[a].core2[r];
+        }
+    }
+}
+
+std::vector<long double> segment_window(const Axis &a, int r) {
+    std::vector<long double> W(a.L, 1.0L);
+    if (a.P > 1) {
+        if (r > 0) {
+            W[0] = 0.0L;
+            for (int m = 1; m <= PSTD_NWEIGHT; ++m) W[m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1));
+        }
+        if (r < a.P - 1) {
+            W[a.L - 1] = 0.0L;
+            for (int m = 1; m <= PSTD_NWEIGHT; ++m) W[a.L - 1 - m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1));
+        }
+    }
+    return W;
+}
+
+// kappa_q for the local line (utility.cpp:80-93), long double
+static std::vector<long double> kappa(long L, double d) {
+    std::vector<long double> k(L);
+    for (long q = 0; q < L; ++q) {
+        long double f = (long double)q / (long double)L;
+        if (q > (L >> 1)) f -= 1.0L;
+        k[q] = 2.0L * PI_L / (long double)d * f;
+    }
+    return k;
+}
+
+// D2[i][j] = (1/L) sum_q kappa_q^2 cos(2 pi q (i-j)/L)   (real, symmetric)
+static std::vector<long double> d2_row_table(long L, double d) {
+    std::vector<long double> k = kappa(L, d), t(L);
+    for (long s = 0; s < L; ++s) {
+        long double acc = 0.0L;
+        for (long q = 0; q < L; ++q) acc += k[q] * k[q] * cosl(2.0L * PI_L * (long double)((q * s) % L) / (long double)L);
+        t[s] = acc / (long double)L;
+    }
+    return t;  // D2[i][j] = t[(i - j) mod L]
+}
+
+// D1[i][j] = Re (1/L) sum_q i kappa_q e^{i kappa_q d/2} e^{2 pi i q (i-j)/L}
+static std::vector<long double> d1_row_table(long L, double d) {
+    std::vector<long double> k = kappa(L, d), t(L);
+    for (long s = 0; s < L; ++s) {
+        long double acc = 0.0L;
+        for (long q = 0; q < L; ++q) {
+            long double ph = k[q] * (long double)d / 2.0L + 2.0L * PI_L * (long double)((q * s) % L) / (long double)L;
+            acc += -k[q] * sinl(ph);   // Re(i k e^{i ph})
+        }
+        t[s] = acc / (long double)L;
+    }
+    return t;
+}
+
+OpLayout op_layout(const Axis &a) {
+    OpLayout lay;
+    long maxc = 0;
+    for (int r = 0; r < (int)a.P; ++r) maxc = std::max(maxc, a.ncore(r));
+    lay.MB = (int)((maxc + 15) / 16);
+    lay.KB = (int)((a.L + 3) / 4);
+    lay.off.resize(a.P);
+    for (int r = 0; r < (int)a.P; ++r) lay.off[r] = (long)r * lay.MB * lay.KB * 64;
+    lay.total = (long)a.P * lay.MB * lay.KB * 64;
+    return lay;
+}
+
+void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout &lay,
+                               std::vector<long double> &frag) {
+    frag.assign(lay.total, 0.0L);
+    std::vector<long double> t = d2_row_table(a.L, a.d);
+    for (int r = 0; r < (int)a.P; ++r) {
+        std::vector<long double> W = segment_window(a, r);
+        long nc = a.ncore(r), i1 = a.i1(r);
+        for (int mb = 0; mb < lay.MB; ++mb)
+            for (int kb = 0; kb < lay.KB; ++kb)
+                for (int lane = 0; lane < 64; ++lane) {
+                    long m = 16 * mb + lane % 16, l = 4 * kb + lane / 16;
+                    long double v = 0.0L;
+                    if (m < nc && l < a.L) {
+                        long s = ((i1 + m - l) % a.L + a.L) % a.L;
+                        v = alpha * t[s] * W[l];
+                    }
+                    frag[lay.off[r] + ((long)mb * lay.KB + kb) * 64 + lane] = v;
+                }
+    }
+}
+
+int build_surface_op(const Geometry &g, int face, SurfaceOp &op, std::string &err) {
+    int a = face / 2;
+    const Axis &ax = g.ax[a];
+    long v = (face % 2 == 0) ? g.vrtl1[a] : g.vrtl2[a];
+    int r = ax.owner(v);
+    if (r < 0) {
+        err = "virtual surface not inside any core";
+        return PSTD_EINVAL;
+    }
+    long li = v - ax.fft1[r];
+    if (li - 4 < 0 || li + 4 > ax.L - 1 || li - 1 < 0) {
+        err = "virtual-surface FD stencil leaves the local line of its subdomain";
+        return PSTD_EINVAL;
+    }
+    op.seg = r;
+    op.li = li;
+    std::vector<long double> W = segment_window(ax, r);
+    std::vector<long double> t = d1_row_table(ax.L, ax.d);
+    op.ps.assign(ax.L, 0.0);
+    op.fd.assign(ax.L, 0.0);
+    for (long l = 0; l < ax.L; ++l) {
+        long s0 = ((li - 1 - l) % ax.L + ax.L) % ax.L, s1 = ((li - l) % ax.L + ax.L) % ax.L;
+        op.ps[l] = (double)(0.5L * (t[s0] + t[s1]) * W[l]);
+    }
+    for (int m = 1; m <= 4; ++m) {
+        op.fd[li + m] += (double)((long double)FDC[m - 1] * W[li + m] / (long double)ax.d);
+        op.fd[li - m] -= (double)((long double)FDC[m - 1] * W[li - m] / (long double)ax.d);
+    }
+    return PSTD_OK;
+}
+
+std::vector<double> abc_vector(const Geometry &g, int a, double dtau, double U0, double alpha) {
+    long N = g.ax[a].N;
+    std::vector<double> v(N);
+    for (long i = 0; i < N; ++i) {
+        double tmp = (i < (N >> 1)) ? std::cosh(alpha * (double)i) : std::cosh(alpha * (double)(N - 1 - i));
+        v[i] = 1.0 - dtau * U0 / (tmp * tmp);
+    }
+    return v;
+}
+
+TfsfAxis tfsf_axis(const Geometry &g, int a, double dtau, long n_ksi) {
+    const long N = g.ax[a].N;
+    const double n0t1 = (double)(n_ksi - 1);
+    const double l = n0t1 * g.ax[a].d;
+    const double coef = 2.0 * dtau / l;
+    const double kh = g.khat[a];
+    TfsfAxis t;
+    t.P.assign(N, 0.0);
+    t.A.assign(N, 0.0);
+    t.E.resize(N);
+    const double step = g.ax[a].d * kh;
+    for (long i = 0; i < N; ++i) {
+        double ph = (double)(i - g.inj[a]) * step;
+        t.E[i] = std::complex<double>(std::cos(ph), std::sin(ph));
+        if (i < g.ksi1a[a] || i > g.ksi2b[a]) continue;
+        double s1 = (double)(i - g.ksi1a[a]) / n0t1, s2 = (double)(g.ksi2b[a] - i) / n0t1;
+        t.P[i] = ksi(s1) * ksi(s2);
+        std::complex<double> A = 0.0;
+        if (i <= g.ksi1b[a]) A += coef * std::complex<double>(d2ksi(s1) / l, 2.0 * dksi(s1) * kh);
+        if (i >= g.ksi2a[a]) A += coef * std::complex<double>(d2ksi(s2) / l, -2.0 * dksi(s2) * kh);
+        t.A[i] = A;
+    }
+    return t;
+}
+
+void initial_field(const Geometry &g, const pstd_params &p, std::vector<double> &psi0,
+                   std::vector<double> &psi1) {
+    const long Nx = g.ax[0].N, Ny = g.ax[1].N, Nz = g.ax[2].N;
+    psi0.assign((size_t)Nx * Ny * Nz * 2, 0.0);
+    psi1.assign((size_t)Nx * Ny * Nz * 2, 0.0);
+    const double n0t1 = (double)(p.n_ksi - 1);
+    double step[3];
+    std::vector<double> prof[3];
+    for (int a = 0; a < 3; ++a) {
+        step[a] = p.d[a] * g.khat[a];
+        prof[a].assign(g.ax[a].N, 0.0);
+        for (long i = g.ksi1a[a]; i <= g.ksi2b[a]; ++i) {
+            double t = 1.0;
+            if (i <= g.ksi1b[a]) t *= ksi((double)(i - g.ksi1a[a]) / n0t1);
+            if (i >= g.ksi2a[a]) t *= ksi((double)(g.ksi2b[a] - i) / n0t1);
+            prof[a][i] = t;
+        }
+    }
+    const double dtc_r = std::cos(p.dtau), dtc_i = -std::sin(p.dtau);
+    for (long i = g.ksi1a[0]; i <= g.ksi2b[0]; ++i)
+        for (long j = g.ksi1a[1]; j <= g.ksi2b[1]; ++j)
+            for (long k = g.ksi1a[2]; k <= g.ksi2b[2]; ++k) {
+                double r = (double)(i - g.inj[0]) * step[0] + (double)(j - g.inj[1]) * step[1] +
+                           (double)(k - g.inj[2]) * step[2];
+                double cr = std::cos(r), sr = std::sin(r);
+                double X = prof[0][i] * prof[1][j] * prof[2][k];
+                size_t o = (((size_t)i * Ny + j) * Nz + k) * 2;
+                psi0[o] = cr * X;
+                psi0[o + 1] = sr * X;
+                psi1[o] = (cr * dtc_r - sr * dtc_i) * X;
+                psi1[o + 1] = (cr * dtc_i + sr * dtc_r) * X;
+            }
+}
+
+long vbox_count(const Geometry &g) {
+    long n = 1;
+    for (int a = 0; a < 3; ++a) {
+        long w = g.vbox2[a] - g.vbox1[a] + 1;
+        n *= (w > 0 ? w : 0);
+    }
+    return n;
+}
+
+void sample_potential(const Geometry &g, const pstd_params &p, pstd_potential_fn V,
+                      std::vector<double> &out) {
+    long n[3];
+    for (int a = 0; a < 3; ++a) n[a] = std::max(0L, g.vbox2[a] - g.vbox1[a] + 1);
+    out.assign((size_t)n[0] * n[1] * n[2], 0.0);
+    if (!V) return;
+    #pragma omp parallel for
+    for (long i = 0; i < n[0]; ++i) {
+        double x = ((double)(g.vbox1[0] + i) - g.orig[0]) * p.d[0];
+        for (long j = 0; j < n[1]; ++j) {
+            double y = ((double)(g.vbox1[1] + j) - g.orig[1]) * p.d[1];
+            for (long k = 0; k < n[2]; ++k) {
+                double z = ((double)(g.vbox1[2] + k) - g.orig[2]) * p.d[2];
+                out[((size_t)i * n[1] + j) * n[2] + k] = V(x, y, z);
+            }
+        }
+    }
+}
+
+// utility.cpp:108-120: skip comment lines (#, %, /) and blank lines
+static bool getaline(std::ifstream &ifs, std::string &line) {
+    line.clear();
+    while (std::getline(ifs, line)) {
+        if (!line.empty() && (line[0] == '#' || line[0] == '%' || line[0] == '/')) continue;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        return true;
+    }
+    line.clear();
+    return false;
+}
+
+int load_input(const char *path, pstd_params &o, std::string &err) {
+    std::ifstream ifs(path ? path : "");
+    if (!ifs.is_open()) {
+        err = std::string("Cannot open input file ") + (path ? path : "(null)");
+        return PSTD_EIO;
+    }
+    std::memset(&o, 0, sizeof(o));
+    o.precision = PSTD_FP64;
+    std::string s;
+    int count = 0;
+    bool ok = true;
+    auto field = [&](auto fn) {
+        if (!getaline(ifs, s)) return;
+        std::istringstream in(s);
+        fn(in);
+        ok = ok && !in.fail();
+        ++count;
+    };
+    field([&](std::istringstream &in) { in >> o.nprocs[0] >> o.nprocs[1] >> o.nprocs[2]; });
+    field([&](std::istringstream &in) { in >> o.dtau >> o.d[0] >> o.d[1] >> o.d[2]; });
+    field([&](std::istringstream &in) { in >> o.L[0] >> o.L[1] >> o.L[2]; });
+    field([&](std::istringstream &in) { in >> o.n_abc >> o.n_sca >> o.n_ksi >> o.n_halo; });
+    field([&](std::istringstream &in) { in >> o.theta >> o.phi; });
+    field([&](std::istringstream &in) { in >> o.E0; });
+    field([&](std::istringstream &in) { in >> o.cap_U0; });
+    field([&](std::istringstream &in) { in >> o.cap_alpha; });
+    field([&](std::istringstream &in) { in >> o.iterations; });
+    if (getaline(ifs, s)) {
+        // the prefix is the whole line; trailing blanks and a CR are dropped
+        size_t e = s.find_last_not_of(" \t\r");
+        s = s.substr(0, e + 1);
+        std::strncpy(o.prefix, s.c_str(), sizeof(o.prefix) - 1);
+        ++count;
+    }
+    if (count < 10 || !ok) {
+        err = std::string("incomplete parameter file ") + path;
+        return PSTD_EINPUT;
+    }
+    return PSTD_OK;
+}
+
+}  // namespace pstd

@@ -1,0 +1,106 @@
+// Host-side geometry and coefficient tables of the MI355X PSTD core.
+//
+// Everything here runs once at solver creation, in FP64 / long double, and
+// produces the tables the HIP kernels consume.  The decomposition index maths
+// is that of CPSTD_QScat::DomainDecomposition (PSTD_QScat.cpp:270-396); the
+// MPI ranks become "segments" per axis of one global grid.
+#pragma once
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../../include/pstd.h"
+
+namespace pstd {
+
+// One axis of the global grid: N points cut into P overlapping segments of
+// local length L (PSTD_QScat.cpp:277-331).
+struct Axis {
+    long N = 0, L = 0, P = 1, h = 0;
+    double d = 0.0;
+    std::vector<long> fft1, core1, core2;   // global, inclusive
+    long ncore(int r) const { return core2[r] - core1[r] + 1; }
+    long i1(int r) const { return core1[r] - fft1[r]; }   // local core start
+    int owner(long g) const {                            // segment whose core holds g
+        for (int r = 0; r < (int)P; ++r)
+            if (core1[r] <= g && g <= core2[r]) return r;
+        return -1;
+    }
+};
+
+struct Geometry {
+    Axis ax[3];
+    long abc1[3], abc2[3], ksi1a[3], ksi1b[3], ksi2a[3], ksi2b[3];
+    long vrtl1[3], vrtl2[3], vbox1[3], vbox2[3], inj[3];
+    double orig[3];
+    double khat[3];          // sin t cos p, sin t sin p, cos t
+};
+
+// Validation + decomposition.  Returns a pstd error code, message in err.
+int build_geometry(const pstd_params &p, Geometry &g, std::string &err);
+void export_geometry(const Geometry &g, pstd_geometry *out);
+
+// Smooth profile family (utility.cpp:47-106), FP64 with the TINY clamp.
+double ksi(double x);
+double dksi(double x);
+double d2ksi(double x);
+
+// Window of segment r along an axis: plane 0 -> 0 and planes 1..8 ->
+// xi(m/9) on a side that has a neighbour (PSTD_QScat.cpp:104-105, :1554-1603).
+std::vector<long double> segment_window(const Axis &a, int r);
+
+// Operator fragments for the MFMA kernels.  For segment r the real matrix
+//    A[m][l] = alpha * D2[i1 + m][l] * W(l),  m < ncore, l < L
+// (D2 = IFFT diag(kappa^2) FFT, real symmetric circulant) is stored as
+// 16x4 MFMA fragments: frag[(mb*KB + kb)*64 + lane] =
+// A[16 mb + lane%16][4 kb + lane/16], zero padded.
+struct OpLayout {
+    int MB = 0, KB = 0;          // 16-row blocks, 4-column blocks
+    std::vector<long> off;       // fragment offset of each segment
+    long total = 0;
+};
+OpLayout op_layout(const Axis &a);
+void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout &lay,
+                               std::vector<long double> &frag);
+
+// Surface operators of face f = 2a + side along axis a: rows
+//   0: delta at the surface point,
+//   1: (D1[v-1,:] + D1[v,:]) / 2 * W   (PS, PSTD_QScat.cpp:1815-1820),
+//   2: sum_m c_m (delta(v+m) - delta(v-m)) * W / d   (FD8, :1804-1813),
+// each of local length L; also the owning segment and local index.
+struct SurfaceOp {
+    int seg = 0;
+    long li = 0;
+    std::vector<double> ps, fd;   // length L
+};
+int build_surface_op(const Geometry &g, int face, SurfaceOp &op, std::string &err);
+
+// ABC mask per axis (InitABC, PSTD_QScat.cpp:560-608), global index.
+std::vector<double> abc_vector(const Geometry &g, int a, double dtau, double U0, double alpha);
+
+// TF/SF separable tables per axis (InitTS, PSTD_QScat.cpp:617-955):
+//   P(i) two-sided profile, A(i) slab-normal coefficient (complex),
+//   E(i) = exp(i (i - inj) d khat).  Source = E_x E_y E_z
+//   (A_x P_y P_z + P_x A_y P_z + P_x P_y A_z).
+struct TfsfAxis {
+    std::vector<double> P;
+    std::vector<std::complex<double>> A, E;
+};
+TfsfAxis tfsf_axis(const Geometry &g, int a, double dtau, long n_ksi);
+
+// Initial incident field psi(0) = Xi e^{i k.(r - r0)} (InitPsi,
+// PSTD_QScat.cpp:959-1118); psi(dtau) = psi(0) e^{-i dtau}.
+void initial_field(const Geometry &g, const pstd_params &p, std::vector<double> &psi0,
+                   std::vector<double> &psi1);
+
+// Potential box sampled through a hook: V * 2 dtau / E0 (InitV,
+// PSTD_QScat.cpp:1121-1141).
+void sample_potential(const Geometry &g, const pstd_params &p, pstd_potential_fn V,
+                      std::vector<double> &out);
+long vbox_count(const Geometry &g);
+
+// Input file (qscat.cpp:214-297, utility.cpp:108-120).
+int load_input(const char *path, pstd_params &out, std::string &err);
+
+}  // namespace pstd

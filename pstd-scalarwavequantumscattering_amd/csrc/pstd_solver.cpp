@@ -1,0 +1,590 @@
+// C ABI of the MI355X PSTD core (include/pstd.h): solver state in HBM,
+// launch orchestration of one Update, surface capture and the surface files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pstd_host.hpp"
+#include "pstd_kernels.hpp"
+
+using namespace pstd;
+
+namespace {
+
+thread_local std::string g_err;
+const double TWOPI = 6.283185307179586476925286766559005768394;
+
+int fail(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+
+int hipfail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory) ? PSTD_ENOMEM : PSTD_ENODEV;
+}
+
+#define HIPCHK(call, what)                              \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hipfail(e_, what); \
+    } while (0)
+
+}  // namespace
+
+struct pstd_solver {
+    pstd_params p;
+    Geometry g;
+    int device = 0;
+    int cus = 256;
+    size_t max_lds = 64 * 1024;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;               // sizeof(T)
+    void *psi[2] = {nullptr, nullptr};
+    int cur = 0;                  // psi[cur] = psi^n, psi[1-cur] = psi^{n-1}
+    OpLayout lay[3];
+    SegTable seg[3];
+    void *op[3] = {nullptr, nullptr, nullptr};
+    void *gvec[3] = {nullptr, nullptr, nullptr};
+    double *tfP[3] = {nullptr, nullptr, nullptr};
+    double2 *tfA[3] = {nullptr, nullptr, nullptr};
+    double2 *tfE[3] = {nullptr, nullptr, nullptr};
+    void *vbox = nullptr;
+    long nvbox = 0;
+    bool has_v = false;
+    double *surf_w = nullptr;     // 6 faces x (ps, fd) x L_a
+    double2 *surf_acc = nullptr;  // 6 faces x 3 x plane
+    SurfaceOp sop[6];
+    long plane_off[6], plane_n[6];
+    long nTau = 1, nSur = 0;
+    std::vector<void *> allocs;
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    std::vector<int> tkind;
+
+    ~pstd_solver() {
+        for (void *q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename X>
+    int alloc(X *&ptr, size_t bytes, const char *what) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+        if (e != hipSuccess) return hipfail(e, what);
+        allocs.push_back(q);
+        ptr = reinterpret_cast<X *>(q);
+        return 0;
+    }
+    template <typename X>
+    int upload(X *&ptr, const void *host, size_t bytes, const char *what) {
+        int rc = alloc(ptr, bytes, what);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(ptr, host, bytes, hipMemcpyHostToDevice), what);
+        return 0;
+    }
+    long npts() const { return g.ax[0].N * g.ax[1].N * g.ax[2].N; }
+};
+
+namespace {
+
+// cast a host double table to the storage type and upload
+int upload_real(pstd_solver *s, void *&dst, const double *src, size_t n, const char *what) {
+    if (s->esz == 8) return s->upload(dst, src, n * 8, what);
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)src[i];
+    return s->upload(dst, f.data(), n * 4, what);
+}
+
+int upload_real_ld(pstd_solver *s, void *&dst, const std::vector<long double> &src, const char *what) {
+    if (s->esz == 8) {
+        std::vector<double> d(src.begin(), src.end());
+        return s->upload(dst, d.data(), d.size() * 8, what);
+    }
+    std::vector<float> f(src.begin(), src.end());
+    return s->upload(dst, f.data(), f.size() * 4, what);
+}
+
+int select_device(pstd_solver *s) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) return fail(PSTD_ENODEV, "no HIP device visible");
+    HIPCHK(hipGetDevice(&s->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, s->device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PSTD_ENODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950 (MI355X)");
+    s->cus = prop.multiProcessorCount;
+    s->max_lds = (size_t)prop.sharedMemPerBlock;
+    HIPCHK(configure_kernels((int)s->max_lds), "hipFuncSetAttribute");
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    return 0;
+}
+
+int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd_potential_fn hook,
+                  pstd_solver **out) {
+    if (!p || !out) return fail(PSTD_EINVAL, "null argument");
+    *out = nullptr;
+    pstd_solver *s = new (std::nothrow) pstd_solver();
+    if (!s) return fail(PSTD_ENOMEM, "host allocation");
+    s->p = *p;
+    int rc = build_geometry(s->p, s->g, g_err);
+    if (rc) { delete s; return rc; }
+    const Geometry &g = s->g;
+    const long nvb = vbox_count(g);
+    if (vbox_host && (long)nv != nvb) {
+        delete s;
+        return fail(PSTD_EINVAL, "vbox has " + std::to_string(nv) + " values, potential box needs " + std::to_string(nvb));
+    }
+    s->esz = (p->precision == PSTD_FP32) ? 4 : 8;
+    if ((rc = select_device(s))) { delete s; return rc; }
+    for (int a = 0; a < 3; ++a) {
+        s->lay[a] = op_layout(g.ax[a]);
+        size_t lds = (s->esz == 8) ? std::max(plane_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB),
+                                              line_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB))
+                                   : std::max(plane_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB),
+                                              line_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB));
+        if (lds > s->max_lds || s->lay[a].MB > 16) {
+            delete s;
+            return fail(PSTD_EINVAL, "axis " + std::to_string(a) + ": local length L=" + std::to_string(g.ax[a].L) +
+                                         " with core " + std::to_string(16 * s->lay[a].MB) +
+                                         " exceeds the on-chip operator tile of this precision");
+        }
+    }
+
+    // operators, fragments for MFMA
+    const long double alpha = 2.0L * (long double)std::sin(p->dtau);   // PSTD_QScat.cpp:134
+    for (int a = 0; a < 3; ++a) {
+        std::vector<long double> frag;
+        build_laplacian_fragments(g.ax[a], alpha, s->lay[a], frag);
+        if ((rc = upload_real_ld(s, s->op[a], frag, "operator"))) { delete s; return rc; }
+        SegTable &T = s->seg[a];
+        std::memset(&T, 0, sizeof(T));
+        T.n = (int)g.ax[a].P;
+        for (int r = 0; r < T.n; ++r) {
+            T.fft1[r] = (int)g.ax[a].fft1[r];
+            T.core1[r] = (int)g.ax[a].core1[r];
+            T.ncore[r] = (int)g.ax[a].ncore(r);
+            T.mb[r] = (int)((g.ax[a].ncore(r) + 15) / 16);
+            T.opoff[r] = (int)s->lay[a].off[r];
+        }
+        std::vector<double> gv = abc_vector(g, a, p->dtau, p->cap_U0, p->cap_alpha);
+        if ((rc = upload_real(s, s->gvec[a], gv.data(), gv.size(), "abc"))) { delete s; return rc; }
+        TfsfAxis tf = tfsf_axis(g, a, p->dtau, p->n_ksi);
+        if ((rc = s->upload(s->tfP[a], tf.P.data(), tf.P.size() * 8, "tfsf"))) { delete s; return rc; }
+        if ((rc = s->upload(s->tfA[a], tf.A.data(), tf.A.size() * 16, "tfsf"))) { delete s; return rc; }
+        if ((rc = s->upload(s->tfE[a], tf.E.data(), tf.E.size() * 16, "tfsf"))) { delete s; return rc; }
+    }
+    // potential box: V * 2 dtau / E0 (PSTD_QScat.cpp:1136)
+    std::vector<double> vb;
+    if (vbox_host) vb.assign(vbox_host, vbox_host + nv);
+    else sample_potential(g, s->p, hook, vb);
+    s->has_v = false;
+    for (double &v : vb) {
+        v = v * 2.0 * p->dtau / p->E0;
+        if (v != 0.0) s->has_v = true;
+    }
+    s->nvbox = nvb;
+    if ((rc = upload_real(s, s->vbox, vb.data(), vb.size(), "vbox"))) { delete s; return rc; }
+    // initial field (InitPsi): psi[1] = psi(0) = psi^{n-1}, psi[0] = psi(dtau) = psi^n
+    {
+        std::vector<double> f0, f1;
+        initial_field(g, s->p, f0, f1);
+        if ((rc = upload_real(s, s->psi[0], f1.data(), f1.size(), "psi"))) { delete s; return rc; }
+        if ((rc = upload_real(s, s->psi[1], f0.data(), f0.size(), "psi"))) { delete s; return rc; }
+        s->cur = 0;
+    }
+    // surfaces
+    std::vector<double> w;
+    long acc_n = 0;
+    for (int f = 0; f < 6; ++f) {
+        if ((rc = build_surface_op(g, f, s->sop[f], g_err))) { delete s; return rc; }
+        int a = f / 2, b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+        s->plane_n[f] = g.ax[b].N * g.ax[c].N;
+        s->plane_off[f] = acc_n;
+        acc_n += 3 * s->plane_n[f];
+        w.insert(w.end(), s->sop[f].ps.begin(), s->sop[f].ps.end());
+        w.insert(w.end(), s->sop[f].fd.begin(), s->sop[f].fd.end());
+    }
+    if ((rc = s->upload(s->surf_w, w.data(), w.size() * 8, "surface weights"))) { delete s; return rc; }
+    if ((rc = s->alloc(s->surf_acc, acc_n * 16, "surface accumulators"))) { delete s; return rc; }
+    HIPCHK(hipMemset(s->surf_acc, 0, acc_n * 16), "hipMemset");
+    s->nTau = 1;   // PSTD_QScat.cpp:1117
+    s->nSur = 0;
+    HIPCHK(hipDeviceSynchronize(), "init");
+    *out = s;
+    return PSTD_OK;
+}
+
+// Kernel timing: when a timing run is active every launch is bracketed by a
+// pair of events recorded on the solver's stream.
+int tmark(pstd_solver *s, int kind) {
+    if (!s->timing) return 0;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e), "hipEventCreate");
+    HIPCHK(hipEventRecord(e, s->stream), "hipEventRecord");
+    s->tev.push_back(e);
+    s->tkind.push_back(kind);
+    return 0;
+}
+#define TMARK(s, k)                 \
+    do {                            \
+        int rc_ = tmark((s), (k));  \
+        if (rc_) return rc_;        \
+    } while (0)
+
+int grid_for(pstd_solver *s, long tiles, int nseg) {
+    long target = (long)s->cus * 8 / std::max(1, nseg);
+    return (int)std::max(1L, std::min(tiles, std::max(target, 1L)));
+}
+
+template <typename T>
+int step_impl(pstd_solver *s) {
+    const Geometry &g = s->g;
+    const long Nx = g.ax[0].N, Ny = g.ax[1].N, Nz = g.ax[2].N;
+    const T *psi = reinterpret_cast<const T *>(s->psi[s->cur]);
+    T *psi0 = reinterpret_cast<T *>(s->psi[1 - s->cur]);
+    // x pass: rows = x planes, columns = (y, z)
+    {
+        PlaneArgs<T> a;
+        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[0]);
+        a.row_stride = Ny * Nz; a.outer_stride = 0; a.outer_count = 1;
+        a.ncols = (int)(Ny * Nz); a.col_tiles = (a.ncols + 31) / 32;
+        a.L = (int)g.ax[0].L; a.KB = s->lay[0].KB; a.MB = s->lay[0].MB; a.seg = s->seg[0];
+        TMARK(s, 0);
+        HIPCHK(launch_plane<T>(a, grid_for(s, (long)a.col_tiles, a.seg.n), s->stream), "x pass");
+        TMARK(s, 0);
+    }
+    // y pass: rows = y, columns = z, outer = x
+    {
+        PlaneArgs<T> a;
+        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[1]);
+        a.row_stride = Nz; a.outer_stride = Ny * Nz; a.outer_count = (int)Nx;
+        a.ncols = (int)Nz; a.col_tiles = (a.ncols + 31) / 32;
+        a.L = (int)g.ax[1].L; a.KB = s->lay[1].KB; a.MB = s->lay[1].MB; a.seg = s->seg[1];
+        TMARK(s, 1);
+        HIPCHK(launch_plane<T>(a, grid_for(s, (long)a.col_tiles * Nx, a.seg.n), s->stream), "y pass");
+        TMARK(s, 1);
+    }
+    // z pass + potential + TF/SF + ABC
+    {
+        LineArgs<T> a;
+        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[2]);
+        a.nlines = Nx * Ny; a.Ny = (int)Ny; a.Nz = (int)Nz;
+        a.L = (int)g.ax[2].L; a.KB = s->lay[2].KB; a.MB = s->lay[2].MB;
+        a.line_tiles = (int)((a.nlines + 31) / 32);
+        a.seg = s->seg[2];
+        a.vbox = reinterpret_cast<const T *>(s->vbox);
+        for (int d = 0; d < 3; ++d) {
+            a.vb1[d] = (int)g.vbox1[d]; a.vb2[d] = (int)g.vbox2[d];
+            a.tfP[d] = s->tfP[d]; a.tfA[d] = s->tfA[d]; a.tfE[d] = s->tfE[d];
+            a.k1a[d] = (int)g.ksi1a[d]; a.k1b[d] = (int)g.ksi1b[d];
+            a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)g.ksi2b[d];
+            a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
+        }
+        const double t = (double)s->nTau * s->p.dtau;        // PSTD_QScat.cpp:1235
+        const double f = t / TWOPI;                         // :1236
+        a.vfac = (T)((f < 0.997) ? f : 1.0);                 // :1272
+        a.use_v = (s->has_v && s->nvbox > 0) ? 1 : 0;
+        a.sin_t = std::sin(t);                              // :1238-1239
+        a.cos_t = std::cos(t);
+        TMARK(s, 2);
+        HIPCHK(launch_line<T>(a, grid_for(s, (long)a.line_tiles, a.seg.n), s->stream), "z pass");
+        TMARK(s, 2);
+    }
+    s->cur = 1 - s->cur;   // swap (:1441-1443)
+    ++s->nTau;             // :1445
+    return 0;
+}
+
+template <typename T>
+int accumulate_impl(pstd_solver *s) {
+    const Geometry &g = s->g;
+    const long Ny = g.ax[1].N, Nz = g.ax[2].N;
+    const double tau = (double)s->nTau * s->p.dtau;   // PSTD_QScat.cpp:1748-1749
+    const T *psi = reinterpret_cast<const T *>(s->psi[s->cur]);
+    long woff = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        SurfArgs a;
+        a.L = (int)g.ax[ax].L;
+        a.ph_r = std::cos(tau);
+        a.ph_i = std::sin(tau);
+        for (int side = 0; side < 2; ++side) {
+            const int f = 2 * ax + side;
+            SurfFace &F = a.f[side];
+            if (ax == 0) { F.stride_l = Ny * Nz; F.stride_b = Nz; F.stride_c = 1; F.nb = (int)Ny; F.nc = (int)Nz; }
+            else if (ax == 1) { F.stride_l = Nz; F.stride_b = Ny * Nz; F.stride_c = 1; F.nb = (int)g.ax[0].N; F.nc = (int)Nz; }
+            else { F.stride_l = 1; F.stride_b = Ny * Nz; F.stride_c = Nz; F.nb = (int)g.ax[0].N; F.nc = (int)Ny; }
+            F.fft1 = g.ax[ax].fft1[s->sop[f].seg];
+            F.li = (int)s->sop[f].li;
+            F.ps = s->surf_w + woff;
+            F.fd = s->surf_w + woff + g.ax[ax].L;
+            woff += 2 * g.ax[ax].L;
+            F.acc = s->surf_acc + s->plane_off[f];
+        }
+        TMARK(s, 3);
+        HIPCHK(launch_surface<T>(psi, a, (long)a.f[0].nb * a.f[0].nc, s->stream), "surface");
+        TMARK(s, 3);
+    }
+    s->nSur++;
+    return 0;
+}
+
+void unpack_body(pstd_solver *s, int deriv, const std::vector<double2> &acc, double *out) {
+    // body order: x1 psi, x1 dpsi, x2 psi, x2 dpsi, y1..., z2 dpsi
+    const double sc = s->nSur > 0 ? 1.0 / (double)s->nSur : 1.0;
+    long o = 0;
+    for (int f = 0; f < 6; ++f) {
+        const long n = s->plane_n[f];
+        const double2 *base = acc.data() + s->plane_off[f];
+        const double2 *kinds[2] = {base, base + (1 + (deriv == PSTD_DERIV_FD ? 1 : 0)) * n};
+        for (int k = 0; k < 2; ++k)
+            for (long i = 0; i < n; ++i) {
+                out[o++] = kinds[k][i].x * sc;
+                out[o++] = kinds[k][i].y * sc;
+            }
+    }
+}
+
+long body_len(const pstd_solver *s) {
+    long n = 0;
+    for (int f = 0; f < 6; ++f) n += 2 * 2 * s->plane_n[f];
+    return n;
+}
+
+int fetch_acc(pstd_solver *s, std::vector<double2> &acc) {
+    long n = 0;
+    for (int f = 0; f < 6; ++f) n += 3 * s->plane_n[f];
+    acc.resize(n);
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");
+    HIPCHK(hipMemcpy(acc.data(), s->surf_acc, n * 16, hipMemcpyDeviceToHost), "copy surfaces");
+    return 0;
+}
+
+int write_surface_file(pstd_solver *s, const std::string &name, const std::vector<double> &body) {
+    FILE *fp = std::fopen(name.c_str(), "wb");
+    if (!fp) return fail(PSTD_EIO, "Cannot open output file " + name);
+    const Geometry &g = s->g;
+    int precision = 8;   // accumulators and file body are FP64 in both modes
+    long hdr[10] = {g.ax[0].N, g.ax[1].N, g.ax[2].N, g.vrtl1[0], g.vrtl2[0],
+                    g.vrtl1[1], g.vrtl2[1], g.vrtl1[2], g.vrtl2[2], s->p.n_abc};
+    double dh[7] = {s->p.d[0], s->p.d[1], s->p.d[2], g.orig[0], g.orig[1], g.orig[2], s->p.E0};
+    bool ok = std::fwrite(&precision, sizeof(int), 1, fp) == 1 && std::fwrite(hdr, sizeof(long), 10, fp) == 10 &&
+              std::fwrite(dh, sizeof(double), 7, fp) == 7 &&
+              std::fwrite(body.data(), sizeof(double), body.size(), fp) == body.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) return fail(PSTD_EIO, "write failed: " + name);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pstd_last_error(void) { return g_err.c_str(); }
+
+int pstd_load_input(const char *path, pstd_params *out) {
+    if (!out) return fail(PSTD_EINVAL, "null output");
+    return load_input(path, *out, g_err);
+}
+
+int pstd_compute_geometry(const pstd_params *p, pstd_geometry *out) {
+    if (!p || !out) return fail(PSTD_EINVAL, "null argument");
+    Geometry g;
+    int rc = build_geometry(*p, g, g_err);
+    if (rc) return rc;
+    export_geometry(g, out);
+    return PSTD_OK;
+}
+
+int pstd_create(const pstd_params *p, pstd_potential_fn V, pstd_solver **out) {
+    return create_common(p, nullptr, 0, V, out);
+}
+
+int pstd_create_vbox(const pstd_params *p, const double *vbox, size_t nvals, pstd_solver **out) {
+    if (!vbox && nvals) return fail(PSTD_EINVAL, "null vbox");
+    return create_common(p, vbox, nvals, nullptr, out);
+}
+
+void pstd_destroy(pstd_solver *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    delete s;
+}
+
+int pstd_update(pstd_solver *s, long nsteps) {
+    if (!s || nsteps < 0) return fail(PSTD_EINVAL, "bad solver or step count");
+    for (long i = 0; i < nsteps; ++i) {
+        int rc = (s->esz == 8) ? step_impl<double>(s) : step_impl<float>(s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+long pstd_steps_per_period(const pstd_solver *s) {
+    return s ? (long)(TWOPI / s->p.dtau + 0.5) : -1;   // PSTD_QScat.cpp:1733-1736
+}
+
+int pstd_accumulate_surface_terms(pstd_solver *s) {
+    if (!s) return fail(PSTD_EINVAL, "null solver");
+    return (s->esz == 8) ? accumulate_impl<double>(s) : accumulate_impl<float>(s);
+}
+
+int pstd_update_accumulate(pstd_solver *s, long nsteps) {
+    if (!s || nsteps < 0) return fail(PSTD_EINVAL, "bad solver or step count");
+    for (long i = 0; i < nsteps; ++i) {
+        int rc = pstd_update(s, 1);
+        if (rc) return rc;
+        rc = pstd_accumulate_surface_terms(s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+long pstd_surface_count(const pstd_solver *s) { return s ? s->nSur : -1; }
+long pstd_time_index(const pstd_solver *s) { return s ? s->nTau : -1; }
+
+int pstd_get_surfaces(pstd_solver *s, int deriv, double *out, size_t nvals) {
+    if (!s || !out) return fail(PSTD_EINVAL, "null argument");
+    if ((long)nvals != body_len(s)) return fail(PSTD_EINVAL, "surface buffer size mismatch");
+    std::vector<double2> acc;
+    int rc = fetch_acc(s, acc);
+    if (rc) return rc;
+    unpack_body(s, deriv, acc, out);
+    return 0;
+}
+
+int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix) {
+    if (!s || !prefix) return fail(PSTD_EINVAL, "null argument");
+    if (!s->nSur) {   // PSTD_QScat.cpp:2392-2396
+        std::fprintf(stderr, "Warning: Virtual Surface terms empty.  No file to be saved.\n");
+        return 0;
+    }
+    std::vector<double2> acc;
+    int rc = fetch_acc(s, acc);
+    if (rc) return rc;
+    const long n0 = s->nTau - s->nSur;   // PSTD_QScat.cpp:2412
+    std::vector<double> body(body_len(s));
+    unpack_body(s, PSTD_DERIV_PS, acc, body.data());
+    if ((rc = write_surface_file(s, std::string(prefix) + "_PS_" + std::to_string(n0) + ".dat", body))) return rc;
+    unpack_body(s, PSTD_DERIV_FD, acc, body.data());
+    if ((rc = write_surface_file(s, std::string(prefix) + "_FD_" + std::to_string(n0) + ".dat", body))) return rc;
+    return pstd_reset_surface_terms(s);
+}
+
+int pstd_reset_surface_terms(pstd_solver *s) {
+    if (!s) return fail(PSTD_EINVAL, "null solver");
+    long n = 0;
+    for (int f = 0; f < 6; ++f) n += 3 * s->plane_n[f];
+    HIPCHK(hipMemsetAsync(s->surf_acc, 0, n * 16, s->stream), "reset surfaces");
+    s->nSur = 0;
+    return 0;
+}
+
+int pstd_get_psi(pstd_solver *s, int level, double *out, size_t nvals) {
+    if (!s || !out || (level != 0 && level != 1)) return fail(PSTD_EINVAL, "bad argument");
+    const long n = s->npts() * 2;
+    if ((long)nvals != n) return fail(PSTD_EINVAL, "psi buffer size mismatch");
+    void *src = s->psi[level == 0 ? s->cur : 1 - s->cur];
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");
+    if (s->esz == 8) {
+        HIPCHK(hipMemcpy(out, src, n * 8, hipMemcpyDeviceToHost), "copy psi");
+    } else {
+        std::vector<float> f(n);
+        HIPCHK(hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost), "copy psi");
+        for (long i = 0; i < n; ++i) out[i] = f[i];
+    }
+    return 0;
+}
+
+int pstd_set_psi(pstd_solver *s, int level, const double *in, size_t nvals) {
+    if (!s || !in || (level != 0 && level != 1)) return fail(PSTD_EINVAL, "bad argument");
+    const long n = s->npts() * 2;
+    if ((long)nvals != n) return fail(PSTD_EINVAL, "psi buffer size mismatch");
+    void *dst = s->psi[level == 0 ? s->cur : 1 - s->cur];
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");
+    if (s->esz == 8) {
+        HIPCHK(hipMemcpy(dst, in, n * 8, hipMemcpyHostToDevice), "copy psi");
+    } else {
+        std::vector<float> f(in, in + n);
+        HIPCHK(hipMemcpy(dst, f.data(), n * 4, hipMemcpyHostToDevice), "copy psi");
+    }
+    return 0;
+}
+
+int pstd_set_time_index(pstd_solver *s, long ntau) {
+    if (!s) return fail(PSTD_EINVAL, "null solver");
+    s->nTau = ntau;
+    return 0;
+}
+
+int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]) {
+    if (!s || !ms_out || nsteps < 0 || naccum < 0) return fail(PSTD_EINVAL, "bad argument");
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");
+    s->timing = true;
+    s->tev.clear();
+    s->tkind.clear();
+    int rc = pstd_update(s, nsteps);
+    for (long i = 0; i < naccum && !rc; ++i) rc = pstd_update_accumulate(s, 1);
+    s->timing = false;
+    double sum[4] = {0, 0, 0, 0};
+    long cnt[4] = {0, 0, 0, 0};
+    hipError_t e = hipStreamSynchronize(s->stream);
+    for (size_t i = 0; i + 1 < s->tev.size() && e == hipSuccess; i += 2) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, s->tev[i], s->tev[i + 1]);
+        sum[s->tkind[i]] += ms;
+        cnt[s->tkind[i]] += 1;
+    }
+    for (hipEvent_t ev : s->tev) (void)hipEventDestroy(ev);
+    s->tev.clear();
+    s->tkind.clear();
+    if (rc) return rc;
+    if (e != hipSuccess) return hipfail(e, "event timing");
+    for (int k = 0; k < 4; ++k) ms_out[k] = cnt[k] ? sum[k] / (double)cnt[k] : 0.0;
+    return 0;
+}
+
+int pstd_line_operator(const pstd_params *p, int axis, int seg, int kind, double *out, size_t nvals) {
+    if (!p || !out || axis < 0 || axis > 2 || kind < 0 || kind > 2) return fail(PSTD_EINVAL, "bad argument");
+    Geometry g;
+    int rc = build_geometry(*p, g, g_err);
+    if (rc) return rc;
+    const Axis &A = g.ax[axis];
+    if (kind == 0) {
+        if (seg < 0 || seg >= A.P) return fail(PSTD_EINVAL, "segment out of range");
+        const long nc = A.ncore(seg);
+        if ((long)nvals != nc * A.L) return fail(PSTD_EINVAL, "size mismatch");
+        OpLayout lay = op_layout(A);
+        std::vector<long double> frag;
+        build_laplacian_fragments(A, 2.0L * (long double)std::sin(p->dtau), lay, frag);
+        for (long m = 0; m < nc; ++m)
+            for (long l = 0; l < A.L; ++l) {
+                const long mb = m / 16, kb = l / 4, lane = (m % 16) + 16 * (l % 4);
+                out[m * A.L + l] = (double)frag[lay.off[seg] + (mb * lay.KB + kb) * 64 + lane];
+            }
+        return 0;
+    }
+    if ((long)nvals != 2 * A.L) return fail(PSTD_EINVAL, "size mismatch");
+    SurfaceOp op;
+    if ((rc = build_surface_op(g, 2 * axis + (kind - 1), op, g_err))) return rc;
+    for (long l = 0; l < A.L; ++l) {
+        out[l] = op.ps[l];
+        out[A.L + l] = op.fd[l];
+    }
+    return 0;
+}
+
+int pstd_synchronize(pstd_solver *s) {
+    if (!s) return fail(PSTD_EINVAL, "null solver");
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// qscat_amd: the reference's input-file driver (qscat.cpp:48-206) on the
+// MI355X library.  One process, one GPU; every subdomain of the input's
+// process grid runs on the device as a segment of one global grid.
+//
+//   qscat_amd -i FILE [--fp32]
+//
+// The time loop, the one-period accumulation windows (the last period, and
+// the periods ending at steps 20000/40000/60000) and the surface files are
+// those of the reference (qscat.cpp:165-198).  The diagnostic plane dumps
+// (qscat.cpp:160-163, :175-179) are not produced.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/pstd.h"
+
+extern "C" double pstd_default_potential(double x, double y, double z);
+
+static void show_usage(const char *app) {
+    std::printf("Usage: %s -i FILE [--fp32]\nwhere FILE: input file for model parameters\n", app);
+}
+
+int main(int argc, char **argv) {
+    std::string openname;
+    bool fp32 = false;
+    for (int i = 1; i < argc; ++i) {
+        std::string arg = argv[i];
+        if (arg == "-h" || arg == "--help") {
+            show_usage(argv[0]);
+            return 0;
+        }
+        if (arg == "--fp32") fp32 = true;
+        if (arg == "-i") {
+            if (i < argc - 1) openname = argv[i + 1];
+            if (i == argc - 1 || openname[0] == '-') {
+                std::fprintf(stderr, "Error: -i requires an argument\n");
+                show_usage(argv[0]);
+                return 1;
+            }
+        }
+    }
+    if (openname.empty()) {
+        std::fprintf(stderr, "Error: mandatory -i FILE for model paramters\n");
+        show_usage(argv[0]);
+        return 1;
+    }
+    pstd_params p;
+    int rc = pstd_load_input(openname.c_str(), &p);
+    if (rc) {
+        std::fprintf(stderr, "Error %d: %s\n", rc, pstd_last_error());
+        return 1;
+    }
+    if (fp32) p.precision = PSTD_FP32;
+    pstd_solver *s = nullptr;
+    rc = pstd_create(&p, pstd_default_potential, &s);
+    if (rc) {
+        std::fprintf(stderr, "Error %d: %s\n", rc, pstd_last_error());
+        return 1;
+    }
+    std::printf("Process topology (x, y, z): %d, %d, %d (segments of one MI355X grid)\n", p.nprocs[0], p.nprocs[1],
+                p.nprocs[2]);
+    const long Iter = p.iterations;
+    const long period = pstd_steps_per_period(s);
+    const long every = (Iter >> 7) > 0 ? (Iter >> 7) : 1;
+    auto t0 = std::chrono::steady_clock::now();
+    bool accum = false;
+    long iter_th = 0;
+    for (long i = 1; i <= Iter && !rc; ++i) {
+        if (i % every == 0) {
+            pstd_synchronize(s);
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::printf("Iter=%ld, clock time: %g\n", i, el);
+            std::fflush(stdout);
+        }
+        rc = pstd_update(s, 1);
+        if (rc) break;
+        if (!accum) {
+            if (i + period - 1 >= Iter) {
+                iter_th = Iter;
+                accum = true;
+            } else if (i + period == 20001 || i + period == 40001 || i + period == 60001) {
+                iter_th = i + period - 1;
+                accum = true;
+            }
+        }
+        if (accum) {
+            rc = pstd_accumulate_surface_terms(s);
+            if (!rc && i == iter_th) {
+                rc = pstd_save_virtual_surfaces(s, p.prefix);
+                accum = false;
+            }
+        }
+    }
+    if (!rc) rc = pstd_synchronize(s);
+    if (rc) std::fprintf(stderr, "Error %d: %s\n", rc, pstd_last_error());
+    pstd_destroy(s);
+    return rc ? 1 : 0;
+}

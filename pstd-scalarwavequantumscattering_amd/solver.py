@@ -24,7 +24,7 @@ EXPORTS = [
     "pstd_accumulate_surface_terms", "pstd_update_accumulate", "pstd_surface_count",
     "pstd_time_index", "pstd_get_surfaces", "pstd_save_virtual_surfaces",
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
-    "pstd_synchronize", "pstd_last_error",
+    "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
 ]
 
 
@@ -75,6 +75,10 @@ def library():
         L.pstd_set_time_index.argtypes = [vp, l]
         L.pstd_last_error.restype = C.c_char_p
         L.pstd_last_error.argtypes = []
+        L.pstd_time_kernels.restype = i
+        L.pstd_time_kernels.argtypes = [vp, l, l, dp]
+        L.pstd_line_operator.restype = i
+        L.pstd_line_operator.argtypes = [C.POINTER(PstdParams), i, i, i, dp, sz]
         _lib = L
     return _lib
 
@@ -100,6 +104,19 @@ def compute_geometry(p):
     g = PstdGeometry()
     _check(library().pstd_compute_geometry(C.byref(p), C.byref(g)))
     return g
+
+
+def line_operator(p, axis, seg, kind):
+    """Dense line operator of the kernels (see pstd_line_operator)."""
+    g = compute_geometry(p)
+    L = p.L[axis]
+    if kind == 0:
+        nc = g.core2[axis][seg] - g.core1[axis][seg] + 1
+        out = np.zeros((nc, L))
+    else:
+        out = np.zeros((2, L))
+    _check(library().pstd_line_operator(C.byref(p), int(axis), int(seg), int(kind), _dp(out), out.size))
+    return out
 
 
 class PSTDQScat:
@@ -160,6 +177,12 @@ class PSTDQScat:
     def update_accumulate(self, nsteps):
         _check(library().pstd_update_accumulate(self.h, int(nsteps)))
 
+    def time_kernels(self, nsteps, naccum=0):
+        """Average ms per launch of [x pass, y pass, z pass, surface]."""
+        out = np.zeros(4)
+        _check(library().pstd_time_kernels(self.h, int(nsteps), int(naccum), _dp(out)))
+        return out
+
     def synchronize(self):
         _check(library().pstd_synchronize(self.h))
 
@@ -192,5 +215,5 @@ class PSTDQScat:
         _check(library().pstd_set_psi(self.h, int(level), _dp(a), a.size))
 
 
-__all__ = ["PSTDQScat", "PSTDError", "load_input", "compute_geometry", "library",
+__all__ = ["PSTDQScat", "line_operator", "PSTDError", "load_input", "compute_geometry", "library",
            "LIB_PATH", "EXPORTS", "PSTD_DERIV_PS", "PSTD_DERIV_FD", "PSTD_FP64"]

@@ -1,0 +1,112 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+FP64: psi and the surface terms within 1e-10 relative L2 (the north-star
+tolerance for the FP64 path; the two sides use different algorithms for the
+line operator -- dense windowed MFMA GEMM vs. FFT per local line -- so they
+agree to rounding, not bit for bit).  FP32: within 2e-5 of the FP64 oracle
+after a few steps (storage and MFMA in FP32, phases in FP64).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pstd_amd
+from pstd_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(nprocs=(2, 2, 2), L=(32, 32, 32), h=10),
+    dict(nprocs=(3, 1, 2), L=(40, 24, 30), h=9),
+    dict(nprocs=(1, 1, 1), L=(24, 20, 22), h=10),
+    dict(nprocs=(1, 3, 1), L=(36, 44, 28), h=12, theta=2.6, phi=-2.1),
+]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _pair(cfg, precision=pstd_amd.PSTD_FP64):
+    p, V = W.config_tiny(precision=precision, **cfg)
+    print(f"case nprocs={tuple(p.nprocs)} L={tuple(p.L)} h={p.n_halo} prec={precision}", flush=True)
+    return p, V, O.Oracle(p, V), pstd_amd.PSTDQScat(p, V)
+
+
+@pytest.mark.parametrize("cfg", CASES)
+def test_fp64_random_state_one_step(cfg):
+    """From the same random psi^n, psi^{n-1}: one Update agrees to 1e-12."""
+    p, V, o, g = _pair(cfg)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    b = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    for ntau in (9, 1450):
+        o.set_psi(0, a); o.set_psi(1, b); o.set_time_index(ntau)
+        g.set_psi(0, a); g.set_psi(1, b); g.set_time_index(ntau)
+        o.update(1)
+        g.Update(1)
+        assert g.time_index() == o.time_index() == ntau + 1
+        assert rel(g.get_psi(0), o.get_psi(0)) < 1e-12
+        assert np.array_equal(g.get_psi(1), a)
+
+
+@pytest.mark.parametrize("cfg", CASES)
+def test_fp64_trajectory_and_surfaces(cfg):
+    p, V, o, g = _pair(cfg)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-14
+    assert rel(g.get_psi(1), o.get_psi(1)) < 1e-14
+    o.update(40)
+    g.Update(40)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-10
+    assert rel(g.get_psi(1), o.get_psi(1)) < 1e-10
+    o.update_accumulate(12)
+    g.update_accumulate(12)
+    assert g.surface_count() == o.surface_count() == 12
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 1e-10
+
+
+def test_fp64_surface_files_match(tmp_path):
+    p, V, o, g = _pair(CASES[0])
+    o.update(5); g.Update(5)
+    o.update_accumulate(7); g.update_accumulate(7)
+    o.save_virtual_surfaces(str(tmp_path / "o"))
+    g.SaveVirtualSurfaces(str(tmp_path / "g"))
+    assert g.surface_count() == 0
+    from pstd_amd.surface_io import read_surface_file
+    n0 = o.time_index() - 7
+    for kind in ("PS", "FD"):
+        ho, do = read_surface_file(tmp_path / f"o_{kind}_{n0}.dat")
+        hg, dg = read_surface_file(tmp_path / f"g_{kind}_{n0}.dat")
+        assert ho == hg
+        assert rel(dg, do) < 1e-10
+
+
+@pytest.mark.parametrize("cfg", CASES[:2])
+def test_fp32_against_fp64_oracle(cfg):
+    p, V, o, g = _pair(cfg, precision=pstd_amd.PSTD_FP32)
+    o.update(30)
+    g.Update(30)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 2e-5
+    o.update_accumulate(8)
+    g.update_accumulate(8)
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 2e-5
+
+
+def test_well96_fp64_parity():
+    """The 96^3 square-well workload (2x1x1, L = 64/96/96): 150 steps, then a
+    30-step accumulation, FP64 path against the oracle."""
+    p, V = W.config_well96()
+    print("case well96", flush=True)
+    o = O.Oracle(p, V)
+    g = pstd_amd.PSTDQScat(p, V)
+    o.update(150)
+    g.Update(150)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-10
+    o.update_accumulate(30)
+    g.update_accumulate(30)
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 1e-10

@@ -1,0 +1,159 @@
+"""CPU tests of the product library's host side through its C ABI: the
+exported symbols, the input-file parser (qscat.cpp:214-297), the geometry
+(PSTD_QScat.cpp:270-396) against the oracle, the operator tables the kernels
+apply against a dense numpy restatement, and the error codes."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pstd_amd
+from pstd_amd import workloads as W
+from oracle import oracle as O
+import bruteforce as BF
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "pstd.h").read_text()
+    names = set(re.findall(r"\b(pstd_[a-z_0-9]+)\s*\(", header))
+    names -= {"pstd_potential_fn"}
+    lib = pstd_amd.library()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(pstd_amd.solver.EXPORTS) <= names
+
+
+def test_load_input_example():
+    p = pstd_amd.load_input(GOLD / "InputExample.txt")
+    assert tuple(p.nprocs) == (1, 2, 1)
+    assert p.dtau == pytest.approx(math.pi / 1000, rel=1e-15)
+    assert tuple(p.d) == pytest.approx((2 * math.pi / 20,) * 3, rel=1e-15)
+    assert tuple(p.L) == (288, 160, 288)
+    assert (p.n_abc, p.n_sca, p.n_ksi, p.n_halo) == (40, 41, 12, 15)
+    assert (p.theta, p.phi) == pytest.approx((math.pi / 2, math.pi / 2), rel=1e-15)
+    assert p.E0 == 0.327216805 and p.cap_U0 == 5.0 and p.cap_alpha == 0.1
+    assert p.iterations == 50000
+    assert p.prefix == b"barrier_50K_U5"
+    assert p.precision == pstd_amd.PSTD_FP64
+    g = pstd_amd.compute_geometry(p)
+    assert tuple(g.N) == (288, 290, 288)
+
+
+def test_load_input_errors(tmp_path):
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.load_input(tmp_path / "missing.txt")
+    assert e.value.code == pstd_amd.PSTD_EIO
+    lines = (GOLD / "InputExample.txt").read_text().splitlines()
+    data = [l for l in lines if l and not l.startswith("#")]
+    assert len(data) == 10
+    short = tmp_path / "short.txt"
+    short.write_text("\n".join(data[:7]) + "\n")
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.load_input(short)
+    assert e.value.code == pstd_amd.PSTD_EINPUT
+    # comment styles, blank lines and a CRLF prefix line
+    odd = tmp_path / "odd.txt"
+    body = ["% c", "/ c", "", "   \t"] + data[:9] + ["surf_out \r"]
+    odd.write_text("\n".join(body) + "\n")
+    p = pstd_amd.load_input(odd)
+    assert p.prefix == b"surf_out" and p.iterations == 50000
+
+
+@pytest.mark.parametrize("mk", [
+    lambda: W.config_well96()[0],
+    lambda: W.config_tiny(nprocs=(3, 1, 2), L=(40, 24, 30), h=9, well=None)[0],
+    lambda: W.config_tiny(nprocs=(1, 3, 1), L=(36, 44, 28), h=12, theta=2.6, phi=-2.1, well=None)[0],
+    lambda: W.config_example()[0],
+])
+def test_geometry_matches_oracle(mk):
+    p = mk()
+    g = pstd_amd.compute_geometry(p)
+    o = O.Oracle(p, None)
+    assert tuple(g.N) == o.N
+    for key in ("abc1", "abc2", "ksi1a", "ksi1b", "ksi2a", "ksi2b", "vrtl1", "vrtl2", "inj"):
+        assert tuple(getattr(g, key)) == o.info[key], key
+    segs = {(a, r): None for a in range(3) for r in range(g.n_seg[a])}
+    for rk in range(o.num_ranks()):
+        info = o.rank_info(rk)
+        for a in range(3):
+            r = info["coord"][a]
+            assert g.fft1[a][r] == info["fft1"][a] and g.fft2[a][r] == info["fft2"][a]
+            assert g.core1[a][r] == info["core1"][a] and g.core2[a][r] == info["core2"][a]
+            segs[(a, r)] = True
+    assert all(segs.values())
+
+
+def test_geometry_rejects_bad_inputs():
+    p, _ = W.config_tiny(nprocs=(2, 1, 1), L=(32, 32, 32), h=8, well=None)
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.compute_geometry(p)
+    assert e.value.code == pstd_amd.PSTD_EINVAL
+    p, _ = W.config_tiny(well=None)
+    p.nprocs[1] = 0
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.compute_geometry(p)
+    assert e.value.code == pstd_amd.PSTD_ETOPO
+    p, _ = W.config_tiny(well=None)
+    p.precision = 2
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.compute_geometry(p)
+    assert e.value.code == pstd_amd.PSTD_EINVAL
+
+
+@pytest.mark.parametrize("cfg", [dict(nprocs=(3, 1, 2), L=(40, 24, 30), h=9),
+                                 dict(nprocs=(4, 4, 4), L=(80, 80, 80), h=16, n_abc=14, n_sca=7, n_ksi=12)])
+def test_laplacian_operator_tables(cfg):
+    """The kernels' alpha * D2 * W rows equal the dense restatement built from
+    numpy's FFT of unit vectors (1e-13 relative)."""
+    p, _ = W.config_tiny(well=None, **cfg)
+    alpha = 2 * math.sin(p.dtau)
+    for a in range(3):
+        D = BF.d2_matrix(p.L[a], p.d[a])
+        for r, (f1, c1, c2) in enumerate(BF.segments(p.nprocs[a], p.L[a], p.n_halo)):
+            A = pstd_amd.line_operator(p, a, r, 0)
+            ref = alpha * D[c1 - f1:c2 - f1 + 1] * BF.window(p.nprocs[a], p.L[a], r)[None, :]
+            assert A.shape == ref.shape
+            assert np.abs(A - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_surface_operator_tables():
+    p, _ = W.config_tiny(nprocs=(2, 1, 3), L=(36, 30, 40), h=10, well=None)
+    g = pstd_amd.compute_geometry(p)
+    FD = (4 / 5, -1 / 5, 4 / 105, -1 / 280)
+    for a in range(3):
+        D1 = BF.d1_matrix(p.L[a], p.d[a])
+        for side in (0, 1):
+            v = (g.vrtl1 if side == 0 else g.vrtl2)[a]
+            r = [rr for rr in range(g.n_seg[a]) if g.core1[a][rr] <= v <= g.core2[a][rr]][0]
+            li = v - g.fft1[a][r]
+            Wv = BF.window(p.nprocs[a], p.L[a], r)
+            ops = pstd_amd.line_operator(p, a, 0, 1 + side)
+            ps_ref = 0.5 * (D1[li - 1] + D1[li]) * Wv
+            assert np.abs(ops[0] - ps_ref).max() <= 1e-13 * np.abs(ps_ref).max()
+            fd_ref = np.zeros(p.L[a])
+            for m, c in enumerate(FD, start=1):
+                fd_ref[li + m] += c * Wv[li + m] / p.d[a]
+                fd_ref[li - m] -= c * Wv[li - m] / p.d[a]
+            assert np.allclose(ops[1], fd_ref, rtol=1e-15, atol=1e-15)
+
+
+def _gpu_present():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")
+def test_create_without_gpu_fails_loudly():
+    p, V = W.config_tiny()
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.PSTDQScat(p, V)
+    assert e.value.code == pstd_amd.PSTD_ENODEV
