@@ -1,0 +1,153 @@
+"""Benchmark of the PSTD_QScat time-stepping core on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a 224^3
+global grid of 4x4x4 overlapping subdomains, local FFT length 80 with a
+16-cell overlap (interior cores 48, edge cores 64), dx = 0.2 at k = 3
+(dx_ref = 0.6), dtau at half the stability limit, a seeded Gaussian-
+correlated random potential (correlation length 1, sigma 0.5, cutoff
+r = 12), incidence along a seeded random direction, 14-cell absorber, FP64.
+(A 256^3 grid with 4x4x4 subdomains of local length 80 has no integer
+overlap in the reference's decomposition N = P (L - 2h) + 2h; L = 80 with
+h = 16 gives 224^3.)
+
+One step = one Update of the whole grid (three windowed local-Fourier
+Laplacian passes with the potential, TF/SF source and absorber fused into
+the last).  value = grid points x steps / seconds, inputs resident in HBM.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one
+GPU; the multi-GPU decomposition is a design only.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+PKG_DIR = ROOT / "pstd-scalarwavequantumscattering_amd"
+HBM_PEAK_GBS = 8000.0
+METRIC = "grid-point·steps/s and time-to-steady-state at 1e-6 surface error vs FP64"
+
+
+def load_pkg():
+    if "pstd_amd" in sys.modules:
+        return sys.modules["pstd_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "pstd_amd", PKG_DIR / "__init__.py", submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["pstd_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def gpu_run(pkg, W, precision, steps, warmup, fn):
+    p, V = fn(precision=precision)
+    s = pkg.PSTDQScat(p, V)
+    npts = s.N[0] * s.N[1] * s.N[2]
+    s.Update(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ms = s.time_kernels(steps, 0)       # HIP events around every launch, same stream
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    acc = s.time_kernels(0, 3)          # surface capture, separately
+    s.close()
+    return p, npts, dt, ms, acc
+
+
+def cpu_baseline(W, steps):
+    """The CPU oracle (FP64 restatement of the reference algorithm, OpenMP
+    over lines) on the same workload, a bounded number of steps."""
+    sys.path.insert(0, str(ROOT))
+    threads = int(os.environ.get("PSTD_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import oracle as orc
+    p, V = W.config_grf224()
+    o = orc.Oracle(p, V)
+    o.update(1)
+    t0 = time.perf_counter()
+    o.update(steps)
+    dt = time.perf_counter() - t0
+    N = o.N
+    del o
+    return {"value": N[0] * N[1] * N[2] * steps / dt, "unit": "grid-point·steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (FP64, OpenMP over lines, {threads} threads), same 224^3 workload, "
+                      f"{steps} steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def traffic_from_profiles(kernel, workload):
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(workload, {}).get(kernel)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    from pstd_amd import workloads as W
+
+    p, npts, dt, ms, acc = gpu_run(pkg, W, pkg.PSTD_FP64, args.steps, args.warmup, W.config_grf224)
+    value = npts * args.steps / dt
+    names = ["x_pass", "y_pass", "z_pass_fused"]
+    dom = max(range(3), key=lambda k: ms[k])
+    bytes_per_pt = 48.0   # read psi^n, read psi^{n-1} accumulator, write it back (FP64)
+    achieved = bytes_per_pt * npts / (ms[dom] * 1e-3) / 1e9
+    step_ms = dt / args.steps * 1e3
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-point·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "grf224: 224^3 grid, 4x4x4 subdomains, L=80, h=16, Gaussian random "
+                               "potential, k=3, FP64",
+                   "global_grid": [int(v) for v in (p.nprocs[0] * (p.L[0] - 2 * p.n_halo) + 2 * p.n_halo,) * 3],
+                   "subdomains": list(p.nprocs), "local_fft_length": list(p.L), "halo": p.n_halo},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(names[dom], "grf224_fp64")},
+        "kernel_ms": {names[k]: ms[k] for k in range(3)} | {"surface_capture_per_axis": acc[3]},
+        "step_roofline_frac": bytes_per_pt * npts / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    if not args.no_fp32:
+        extra = {}
+        for name, fn in (("grf224_fp32", W.config_grf224), ("wells320_fp32", W.config_wells320)):
+            _, n2, dt2, ms2, _ = gpu_run(pkg, W, pkg.PSTD_FP32, max(50, args.steps // 4), 10, fn)
+            extra[name] = {"value": n2 * max(50, args.steps // 4) / dt2,
+                           "ms_per_step": dt2 / max(50, args.steps // 4) * 1e3,
+                           "kernel_ms": [float(v) for v in ms2[:3]]}
+        line["fp32_fp64phase"] = extra
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

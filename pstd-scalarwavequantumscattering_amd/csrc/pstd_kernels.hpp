@@ -74,8 +74,8 @@ struct SurfArgs {          // the two faces of one axis
 template <typename T> hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s);
 template <typename T> hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s);
 template <typename T> hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStream_t s);
-template <typename T> size_t plane_lds_bytes(int MB, int KB);
-template <typename T> size_t line_lds_bytes(int MB, int KB);
+template <typename T> size_t pass_lds_bytes(int MB, int KB);
+template <typename T> int pass_occupancy(int MB, int KB, bool line);
 hipError_t configure_kernels(int max_lds);
 
 }  // namespace pstd

@@ -45,6 +45,7 @@ struct pstd_solver {
     int device = 0;
     int cus = 256;
     size_t max_lds = 64 * 1024;
+    int occ[3] = {1, 1, 1};       // resident blocks per CU of each axis pass
     hipStream_t stream = nullptr;
     size_t esz = 8;               // sizeof(T)
     void *psi[2] = {nullptr, nullptr};
@@ -146,16 +147,17 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
     if ((rc = select_device(s))) { delete s; return rc; }
     for (int a = 0; a < 3; ++a) {
         s->lay[a] = op_layout(g.ax[a]);
-        size_t lds = (s->esz == 8) ? std::max(plane_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB),
-                                              line_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB))
-                                   : std::max(plane_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB),
-                                              line_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB));
+        size_t lds = (s->esz == 8) ? pass_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB)
+                                   : pass_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB);
         if (lds > s->max_lds || s->lay[a].MB > 16) {
             delete s;
             return fail(PSTD_EINVAL, "axis " + std::to_string(a) + ": local length L=" + std::to_string(g.ax[a].L) +
                                          " with core " + std::to_string(16 * s->lay[a].MB) +
                                          " exceeds the on-chip operator tile of this precision");
         }
+        const bool line = (a == 2);
+        s->occ[a] = (s->esz == 8) ? pass_occupancy<double>(s->lay[a].MB, s->lay[a].KB, line)
+                                  : pass_occupancy<float>(s->lay[a].MB, s->lay[a].KB, line);
     }
 
     // operators, fragments for MFMA
@@ -239,9 +241,16 @@ int tmark(pstd_solver *s, int kind) {
         if (rc_) return rc_;        \
     } while (0)
 
-int grid_for(pstd_solver *s, long tiles, int nseg) {
-    long target = (long)s->cus * 8 / std::max(1, nseg);
-    return (int)std::max(1L, std::min(tiles, std::max(target, 1L)));
+// Persistent grid width per segment: about four blocks per CU in total, a
+// multiple of 8 so that block (b, r) and (b, r+1) -- the two segments that
+// share an overlap of the same columns -- land on the same XCD (blocks are
+// dealt to the 8 XCDs round robin) and meet in its L2.
+// The width covers exactly the resident blocks (occupancy x CUs) so that no
+// partial second round of blocks is left.
+int grid_for(pstd_solver *s, int axis, long tiles, int nseg) {
+    long target = (long)s->cus * s->occ[axis] / std::max(1, nseg);
+    target = std::max(8L, target / 8 * 8);
+    return (int)std::max(1L, std::min(tiles, target));
 }
 
 template <typename T>
@@ -258,7 +267,7 @@ int step_impl(pstd_solver *s) {
         a.ncols = (int)(Ny * Nz); a.col_tiles = (a.ncols + 31) / 32;
         a.L = (int)g.ax[0].L; a.KB = s->lay[0].KB; a.MB = s->lay[0].MB; a.seg = s->seg[0];
         TMARK(s, 0);
-        HIPCHK(launch_plane<T>(a, grid_for(s, (long)a.col_tiles, a.seg.n), s->stream), "x pass");
+        HIPCHK(launch_plane<T>(a, grid_for(s, 0, (long)a.col_tiles, a.seg.n), s->stream), "x pass");
         TMARK(s, 0);
     }
     // y pass: rows = y, columns = z, outer = x
@@ -269,7 +278,7 @@ int step_impl(pstd_solver *s) {
         a.ncols = (int)Nz; a.col_tiles = (a.ncols + 31) / 32;
         a.L = (int)g.ax[1].L; a.KB = s->lay[1].KB; a.MB = s->lay[1].MB; a.seg = s->seg[1];
         TMARK(s, 1);
-        HIPCHK(launch_plane<T>(a, grid_for(s, (long)a.col_tiles * Nx, a.seg.n), s->stream), "y pass");
+        HIPCHK(launch_plane<T>(a, grid_for(s, 1, (long)a.col_tiles * Nx, a.seg.n), s->stream), "y pass");
         TMARK(s, 1);
     }
     // z pass + potential + TF/SF + ABC
@@ -295,7 +304,7 @@ int step_impl(pstd_solver *s) {
         a.sin_t = std::sin(t);                              // :1238-1239
         a.cos_t = std::cos(t);
         TMARK(s, 2);
-        HIPCHK(launch_line<T>(a, grid_for(s, (long)a.line_tiles, a.seg.n), s->stream), "z pass");
+        HIPCHK(launch_line<T>(a, grid_for(s, 2, (long)a.line_tiles, a.seg.n), s->stream), "z pass");
         TMARK(s, 2);
     }
     s->cur = 1 - s->cur;   // swap (:1441-1443)

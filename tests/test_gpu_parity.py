@@ -22,6 +22,10 @@ CASES = [
     dict(nprocs=(3, 1, 2), L=(40, 24, 30), h=9),
     dict(nprocs=(1, 1, 1), L=(24, 20, 22), h=10),
     dict(nprocs=(1, 3, 1), L=(36, 44, 28), h=12, theta=2.6, phi=-2.1),
+    # L = 80 (the 224^3 / 320^3 workloads' local length), interior segment
+    dict(nprocs=(3, 1, 1), L=(80, 24, 24), h=16, theta=1.1, phi=0.4),
+    # L = 112 > 96: the LDS-operator kernel variant
+    dict(nprocs=(2, 1, 1), L=(112, 24, 24), h=16, theta=0.2, phi=3.0),
 ]
 
 
