@@ -45,6 +45,12 @@ extern "C" {
 #define PSTD_FP64 8
 #define PSTD_FP32 4
 
+/* Line-operator method of an axis pass: the windowed local-Fourier second
+ * derivative as a dense real operator on MFMA (GEMM), or as an LDS Stockham
+ * FFT pair with the spectral multiplier (FFT, L = 2^a 3^b 5^c 7^d). */
+#define PSTD_METHOD_GEMM 0
+#define PSTD_METHOD_FFT  1
+
 /* Derivative selector for surface output; PSTD_QScat.h:53 (DerivativeType). */
 #define PSTD_DERIV_PS 0
 #define PSTD_DERIV_FD 1
@@ -198,6 +204,14 @@ int pstd_synchronize(pstd_solver *s);
  * pstd_update / pstd_update_accumulate.  No reference counterpart (the
  * reference times with MPI_Wtime, qscat.cpp:165-172). */
 int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]);
+
+/* Line-operator method of axis a (0 x, 1 y, 2 z): PSTD_METHOD_GEMM or
+ * PSTD_METHOD_FFT.  pstd_create picks GEMM where its tiles fit on chip,
+ * else FFT; pstd_set_method returns 22 if the method is unavailable for
+ * that axis (FFT needs L = 2^a 3^b 5^c 7^d <= 512).  No reference
+ * counterpart: the reference always uses MKL DFTI (PSTD_QScat.cpp:496-526). */
+int pstd_set_method(pstd_solver *s, int axis, int method);
+int pstd_get_method(const pstd_solver *s, int axis);
 
 /* The dense real line operator the kernels apply for axis a and segment r:
  * kind 0: alpha * D2[core rows][0..L) * W (ncore x L, row-major), the

@@ -233,6 +233,42 @@ void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout 
     }
 }
 
+FftTables build_fft_tables(const Axis &a, long double alpha) {
+    FftTables t;
+    long m = a.L;
+    std::vector<int> rad;
+    while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+    while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+    for (int p : {3, 5, 7})
+        while (m % p == 0) { rad.push_back(p); m /= p; }
+    if (m != 1 || rad.size() > 16 || rad.empty()) return t;
+    t.nst = (int)rad.size();
+    long Ns = 1, off = 0;
+    for (int s = 0; s < t.nst; ++s) {
+        const int P = rad[s];
+        t.rad[s] = P;
+        t.ns[s] = (int)Ns;
+        t.twoff[s] = (int)off;
+        for (long k = 0; k < Ns; ++k)
+            for (int q = 1; q < P; ++q) {
+                const long double ang = -2.0L * PI_L * (long double)(q * k) / (long double)(Ns * P);
+                t.tw.push_back(cosl(ang));
+                t.tw.push_back(sinl(ang));
+            }
+        off += Ns * (P - 1);
+        Ns *= P;
+    }
+    std::vector<long double> k = kappa(a.L, a.d);
+    t.kmul.resize(a.L);
+    for (long l = 0; l < a.L; ++l) t.kmul[l] = alpha * k[l] * k[l] / (long double)a.L;
+    for (int r = 0; r < (int)a.P; ++r) {
+        std::vector<long double> W = segment_window(a, r);
+        t.win.insert(t.win.end(), W.begin(), W.end());
+    }
+    t.ok = true;
+    return t;
+}
+
 int build_surface_op(const Geometry &g, int face, SurfaceOp &op, std::string &err) {
     int a = face / 2;
     const Axis &ax = g.ax[a];
@@ -288,12 +324,15 @@ TfsfAxis tfsf_axis(const Geometry &g, int a, double dtau, long n_ksi) {
     for (long i = 0; i < N; ++i) {
         double ph = (double)(i - g.inj[a]) * step;
         t.E[i] = std::complex<double>(std::cos(ph), std::sin(ph));
-        if (i < g.ksi1a[a] || i > g.ksi2b[a]) continue;
         double s1 = (double)(i - g.ksi1a[a]) / n0t1, s2 = (double)(g.ksi2b[a] - i) / n0t1;
-        t.P[i] = ksi(s1) * ksi(s2);
+        // the profile of the other axes' slabs lives on the TF box
+        // [ksi1a, ksi2b]; the slab-normal coefficient on the slab ranges
+        // [ksi1a, ksi1b] and [ksi2a, ksi2b], which reach past the box when
+        // the box is thinner than two transition layers (:741-771)
+        if (i >= g.ksi1a[a] && i <= g.ksi2b[a]) t.P[i] = ksi(s1) * ksi(s2);
         std::complex<double> A = 0.0;
-        if (i <= g.ksi1b[a]) A += coef * std::complex<double>(d2ksi(s1) / l, 2.0 * dksi(s1) * kh);
-        if (i >= g.ksi2a[a]) A += coef * std::complex<double>(d2ksi(s2) / l, -2.0 * dksi(s2) * kh);
+        if (i >= g.ksi1a[a] && i <= g.ksi1b[a]) A += coef * std::complex<double>(d2ksi(s1) / l, 2.0 * dksi(s1) * kh);
+        if (i >= g.ksi2a[a] && i <= g.ksi2b[a]) A += coef * std::complex<double>(d2ksi(s2) / l, -2.0 * dksi(s2) * kh);
         t.A[i] = A;
     }
     return t;

@@ -64,6 +64,21 @@ OpLayout op_layout(const Axis &a);
 void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout &lay,
                                std::vector<long double> &frag);
 
+// Stockham FFT tables of an axis (FFT variant of the passes): radices
+// (4, 2, 3, 5, 7 in that order), Ns per stage, twiddles
+// e^{-2 pi i q k / (Ns P)} at [twoff + k (P-1) + q - 1], the multiplier
+// alpha kappa_l^2 / L and the windows of all segments ([P][L]).  ok is false
+// when L has another prime factor or needs more than 16 stages.
+struct FftTables {
+    bool ok = false;
+    int nst = 0;
+    int rad[16], ns[16], twoff[16];
+    std::vector<long double> tw;     // interleaved complex
+    std::vector<long double> kmul;
+    std::vector<long double> win;
+};
+FftTables build_fft_tables(const Axis &a, long double alpha);
+
 // Surface operators of face f = 2a + side along axis a: rows
 //   0: delta at the surface point,
 //   1: (D1[v-1,:] + D1[v,:]) / 2 * W   (PS, PSTD_QScat.cpp:1815-1820),

@@ -172,6 +172,34 @@ __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
 // potential with its ramp (PSTD_QScat.cpp:1268-1303), TF/SF source
 // (:1306-1419, regenerated from separable FP64 tables and the FP64 time
 // phase) and the ABC mask (:1421-1435).
+// One output point of the fused z pass: psi0 at dst, U = alpha D2 (W psi)
+// at the point (alpha folded), pn = psi^n at the point.
+template <typename T>
+__device__ __forceinline__ void z_point(const LineArgs<T> &a, const LineInfo &li, const ZInfo &zi,
+                                        typename VT<T>::c2 *dst, T ur, T ui, typename VT<T>::c2 pn) {
+    typedef typename VT<T>::c2 C2;
+    C2 acc = *dst;
+    acc.x += ui;
+    acc.y -= ur;
+    if ((zi.flags & 1) && li.vrow >= 0) {
+        const T v = a.vfac * a.vbox[li.vrow + zi.zoff];
+        acc.x += v * pn.y;
+        acc.y -= v * pn.x;
+    }
+    if ((zi.flags & 2) && (li.flags & 1) && !((zi.flags & 4) && (li.flags & 2))) {
+        const double ur2 = li.cxy.x * zi.pz + li.pxy * zi.az.x;
+        const double ui2 = li.cxy.y * zi.pz + li.pxy * zi.az.y;
+        const double er = li.exy.x * zi.ez.x - li.exy.y * zi.ez.y;
+        const double ei = li.exy.x * zi.ez.y + li.exy.y * zi.ez.x;
+        acc.x -= (T)(er * ur2 - ei * ui2);
+        acc.y -= (T)(er * ui2 + ei * ur2);
+    }
+    const T gg = (T)(li.gxy * zi.gz);
+    acc.x *= gg;
+    acc.y *= gg;
+    *dst = acc;
+}
+
 template <typename T>
 __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const typename VT<T>::c2 *Ls, const LineInfo *LI,
                                               int LS, const ZInfo &zi, int z, int m, int i1, int kq,
@@ -186,30 +214,8 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const typena
             const int line = hb * 16 + VT<T>::row(kq, q);
             const LineInfo &li = LI[line];
             if (li.base < 0) continue;
-            C2 *dst = psi0 + li.base + z;
-            C2 acc = *dst;
-            const T ur = hb ? ar1[q] : ar0[q];
-            const T ui = hb ? ai1[q] : ai0[q];
-            acc.x += ui;
-            acc.y -= ur;
-            if ((zi.flags & 1) && li.vrow >= 0) {
-                const C2 pn = Ls[line * LS + i1 + m];
-                const T v = a.vfac * a.vbox[li.vrow + zi.zoff];
-                acc.x += v * pn.y;
-                acc.y -= v * pn.x;
-            }
-            if ((zi.flags & 2) && (li.flags & 1) && !((zi.flags & 4) && (li.flags & 2))) {
-                const double ur2 = li.cxy.x * zi.pz + li.pxy * zi.az.x;
-                const double ui2 = li.cxy.y * zi.pz + li.pxy * zi.az.y;
-                const double er = li.exy.x * zi.ez.x - li.exy.y * zi.ez.y;
-                const double ei = li.exy.x * zi.ez.y + li.exy.y * zi.ez.x;
-                acc.x -= (T)(er * ur2 - ei * ui2);
-                acc.y -= (T)(er * ui2 + ei * ur2);
-            }
-            const T gg = (T)(li.gxy * zi.gz);
-            acc.x *= gg;
-            acc.y *= gg;
-            *dst = acc;
+            z_point<T>(a, li, zi, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
+                       Ls[line * LS + i1 + m]);
         }
     }
 }
@@ -520,10 +526,269 @@ __global__ void __launch_bounds__(256) k_surface(const T *psi_, SurfArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// FFT variant.  A tile of FFT_NC lines sits in LDS as S[l][c] (row stride
+// FFT_NC + 1 to spread banks).  Stockham autosort stages, radix P: butterfly
+// j of a line reads x_q = S[j + q L/P], applies the twiddle w^{q (j mod Ns)},
+// w = e^{-2 pi i/(Ns P)}, and writes DFT_P(x)_s to S[(j/Ns) Ns P + j mod Ns
+// + s Ns]; every butterfly of a stage is read into registers before the
+// block barrier and written after it, so one buffer suffices.
+// ---------------------------------------------------------------------------
+constexpr int FNTHR = 512;
+
+template <typename C2> __device__ __forceinline__ C2 cmul(C2 a, C2 b) {
+    C2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+// forward: y_s = sum_q x_q e^{-2 pi i q s / P}; inverse: e^{+}
+template <typename T, int P>
+__device__ __forceinline__ void dft_small(typename VT<T>::c2 *x, bool inv) {
+    typedef typename VT<T>::c2 C2;
+    const T sg = inv ? T(1) : T(-1);   // sign of the imaginary part of the root
+    if (P == 2) {
+        C2 a = x[0], b = x[1];
+        x[0].x = a.x + b.x; x[0].y = a.y + b.y;
+        x[1].x = a.x - b.x; x[1].y = a.y - b.y;
+    } else if (P == 4) {
+        C2 a, b, c, d;
+        a.x = x[0].x + x[2].x; a.y = x[0].y + x[2].y;
+        b.x = x[0].x - x[2].x; b.y = x[0].y - x[2].y;
+        c.x = x[1].x + x[3].x; c.y = x[1].y + x[3].y;
+        d.x = x[1].x - x[3].x; d.y = x[1].y - x[3].y;
+        // d * (sg i)
+        C2 di;
+        di.x = -sg * d.y;
+        di.y = sg * d.x;
+        x[0].x = a.x + c.x; x[0].y = a.y + c.y;
+        x[2].x = a.x - c.x; x[2].y = a.y - c.y;
+        x[1].x = b.x + di.x; x[1].y = b.y + di.y;
+        x[3].x = b.x - di.x; x[3].y = b.y - di.y;
+    } else if (P == 3) {
+        const T s3 = T(0.86602540378443864676372317075293618347);
+        C2 t, u;
+        t.x = x[1].x + x[2].x; t.y = x[1].y + x[2].y;
+        u.x = x[1].x - x[2].x; u.y = x[1].y - x[2].y;
+        C2 m;
+        m.x = x[0].x - T(0.5) * t.x;
+        m.y = x[0].y - T(0.5) * t.y;
+        // (sg i) s3 u
+        C2 w;
+        w.x = -sg * s3 * u.y;
+        w.y = sg * s3 * u.x;
+        x[0].x += t.x; x[0].y += t.y;
+        x[1].x = m.x + w.x; x[1].y = m.y + w.y;
+        x[2].x = m.x - w.x; x[2].y = m.y - w.y;
+    } else {
+        // generic small prime: cos/sin of 2 pi k / P (k < P)
+        T cs[P], sn[P];
+        if (P == 5) {
+            const T c1 = T(0.30901699437494742410229341718281905886), s1 = T(0.95105651629515357211643933337938214340);
+            const T c2 = T(-0.80901699437494742410229341718281905886), s2 = T(0.58778525229247312916870595463907276860);
+            cs[0] = 1; sn[0] = 0; cs[1] = c1; sn[1] = s1; cs[2] = c2; sn[2] = s2;
+            cs[3] = c2; sn[3] = -s2; cs[4] = c1; sn[4] = -s1;
+        } else {   // P == 7
+            const T c1 = T(0.62348980185873353052500488400423981063), s1 = T(0.78183148246802980870844452667405775023);
+            const T c2 = T(-0.22252093395631440428890256449679475946), s2 = T(0.97492791218182360701813168299393121723);
+            const T c3 = T(-0.90096886790241912623610231950744505116), s3 = T(0.43388373911755812047576833284835875461);
+            cs[0] = 1; sn[0] = 0; cs[1] = c1; sn[1] = s1; cs[2] = c2; sn[2] = s2; cs[3] = c3; sn[3] = s3;
+            cs[4] = c3; sn[4] = -s3; cs[5] = c2; sn[5] = -s2; cs[6] = c1; sn[6] = -s1;
+        }
+        C2 y[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            C2 acc = x[0];
+#pragma unroll
+            for (int q = 1; q < P; ++q) {
+                const int k = (q * s) % P;        // root e^{sg i 2 pi k/P}
+                const T wr = cs[k], wi = sg * sn[k];
+                acc.x += x[q].x * wr - x[q].y * wi;
+                acc.y += x[q].x * wi + x[q].y * wr;
+            }
+            y[s] = acc;
+        }
+#pragma unroll
+        for (int s = 0; s < P; ++s) x[s] = y[s];
+    }
+}
+
+template <typename T, int P>
+__device__ void fft_stage(typename VT<T>::c2 *S, int L, int Ns, const typename VT<T>::c2 *tw, bool inv) {
+    typedef typename VT<T>::c2 C2;
+    constexpr int LDW = FFT_NC + 1;
+    constexpr int MAXK = (FFT_LMAX / P * FFT_NC + FNTHR - 1) / FNTHR;
+    const int nb = L / P, items = nb * FFT_NC;
+    C2 y[MAXK][P];
+    int ob[MAXK], oc[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+        const int it = threadIdx.x + k * FNTHR;
+        ob[k] = -1;
+        oc[k] = 0;
+        if (it < items) {
+            const int c = it % FFT_NC, j = it / FFT_NC, kk = j % Ns;
+            C2 x[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) x[q] = S[(j + q * nb) * LDW + c];
+#pragma unroll
+            for (int q = 1; q < P; ++q) {
+                C2 w = tw[kk * (P - 1) + q - 1];
+                if (inv) w.y = -w.y;
+                x[q] = cmul(x[q], w);
+            }
+            dft_small<T, P>(x, inv);
+#pragma unroll
+            for (int q = 0; q < P; ++q) y[k][q] = x[q];
+            ob[k] = (j / Ns) * Ns * P + kk;
+            oc[k] = c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+        if (ob[k] >= 0) {
+#pragma unroll
+            for (int s = 0; s < P; ++s) S[(ob[k] + s * Ns) * LDW + oc[k]] = y[k][s];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void fft_run(typename VT<T>::c2 *S, int L, const FftPlan<T> &f, bool inv) {
+    typedef typename VT<T>::c2 C2;
+    const C2 *tw = reinterpret_cast<const C2 *>(f.tw);
+    for (int st = 0; st < f.nst; ++st) {
+        const C2 *t = tw + f.twoff[st];
+        switch (f.rad[st]) {
+            case 2: fft_stage<T, 2>(S, L, f.ns[st], t, inv); break;
+            case 3: fft_stage<T, 3>(S, L, f.ns[st], t, inv); break;
+            case 4: fft_stage<T, 4>(S, L, f.ns[st], t, inv); break;
+            case 5: fft_stage<T, 5>(S, L, f.ns[st], t, inv); break;
+            default: fft_stage<T, 7>(S, L, f.ns[st], t, inv); break;
+        }
+    }
+}
+
+// forward FFT, x alpha kappa^2 / L, backward FFT, in place on the tile
+template <typename T>
+__device__ __forceinline__ void fft_laplacian(typename VT<T>::c2 *S, int L, const FftPlan<T> &f) {
+    constexpr int LDW = FFT_NC + 1;
+    fft_run<T>(S, L, f, false);
+    for (int i = threadIdx.x; i < L * FFT_NC; i += FNTHR) {
+        const int l = i / FFT_NC, c = i - l * FFT_NC;
+        const T k = f.kmul[l];
+        S[l * LDW + c].x *= k;
+        S[l * LDW + c].y *= k;
+    }
+    __syncthreads();
+    fft_run<T>(S, L, f, true);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(FNTHR) k_fft_plane(PlaneArgs<T> a) {
+    typedef typename VT<T>::c2 C2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int LDW = FFT_NC + 1;
+    C2 *S = reinterpret_cast<C2 *>(smem);
+    const int r = blockIdx.y, L = a.L, tid = threadIdx.x;
+    const long fft1 = a.seg.fft1[r], core1 = a.seg.core1[r];
+    const int ncore = a.seg.ncore[r], i1 = (int)(core1 - fft1);
+    const T *win = a.fft.win + (long)r * L;
+    const C2 *psi = reinterpret_cast<const C2 *>(a.psi);
+    C2 *psi0 = reinterpret_cast<C2 *>(a.psi0);
+    const int ntiles = a.outer_count * a.fcol_tiles;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int o = t / a.fcol_tiles;
+        const long c0 = (long)(t - o * a.fcol_tiles) * FFT_NC;
+        const C2 *src = psi + (long)o * a.outer_stride + fft1 * a.row_stride + c0;
+        __syncthreads();
+        for (int i = tid; i < L * FFT_NC; i += FNTHR) {
+            const int l = i / FFT_NC, c = i - l * FFT_NC;
+            C2 v = czero<C2>();
+            if (c0 + c < a.ncols) {
+                v = src[(long)l * a.row_stride + c];
+                v.x *= win[l];
+                v.y *= win[l];
+            }
+            S[l * LDW + c] = v;
+        }
+        __syncthreads();
+        fft_laplacian<T>(S, L, a.fft);
+        for (int i = tid; i < ncore * FFT_NC; i += FNTHR) {
+            const int m = i / FFT_NC, c = i - m * FFT_NC;
+            if (c0 + c >= a.ncols) continue;
+            const C2 u = S[(i1 + m) * LDW + c];
+            C2 *dst = psi0 + (long)o * a.outer_stride + (core1 + m) * a.row_stride + c0 + c;
+            C2 v = *dst;
+            v.x += u.y;
+            v.y -= u.x;
+            *dst = v;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
+    typedef typename VT<T>::c2 C2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int LDW = FFT_NC + 1;
+    const int L = a.L, tid = threadIdx.x;
+    C2 *S = reinterpret_cast<C2 *>(smem);
+    LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)L * LDW * sizeof(C2)));
+    const int r = blockIdx.y;
+    const long fft1 = a.seg.fft1[r];
+    const int core1 = a.seg.core1[r], ncore = a.seg.ncore[r], i1 = core1 - (int)fft1;
+    const T *win = a.fft.win + (long)r * L;
+    const C2 *psi = reinterpret_cast<const C2 *>(a.psi);
+    C2 *psi0 = reinterpret_cast<C2 *>(a.psi0);
+    for (int t = blockIdx.x; t < a.fline_tiles; t += gridDim.x) {
+        const long q0 = (long)t * FFT_NC;
+        __syncthreads();
+        for (int i = tid; i < L * FFT_NC; i += FNTHR) {
+            const int c = i / L, l = i - c * L;
+            C2 v = czero<C2>();
+            if (q0 + c < a.nlines) {
+                v = psi[(q0 + c) * a.Nz + fft1 + l];
+                v.x *= win[l];
+                v.y *= win[l];
+            }
+            S[l * LDW + c] = v;
+        }
+        if (tid < FFT_NC) line_info<T>(a, q0 + tid, LI[tid]);
+        __syncthreads();
+        fft_laplacian<T>(S, L, a.fft);
+        for (int i = tid; i < ncore * FFT_NC; i += FNTHR) {
+            const int c = i / ncore, m = i - c * ncore;
+            const LineInfo &li = LI[c];
+            if (li.base < 0) continue;
+            const int z = core1 + m;
+            const ZInfo zi = z_info<T>(a, z);
+            const C2 u = S[(i1 + m) * LDW + c];
+            z_point<T>(a, li, zi, psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
+        }
+    }
+}
+
+template <typename T> size_t fft_lds_bytes(int L) {
+    return align16((size_t)L * (FFT_NC + 1) * 2 * sizeof(T)) + FFT_NC * LINEINFO_BYTES;
+}
+
+template <typename T> int fft_occupancy(int L, bool line) {
+    int n = 0;
+    hipError_t e = line ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_line<T>, FNTHR, fft_lds_bytes<T>(L))
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_plane<T>, FNTHR, fft_lds_bytes<T>(L));
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+// ---------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
     dim3 grid(grid_x, a.seg.n);
-    if (reg_variant_ok(a.MB, a.KB)) {
+    if (a.method == PSTD_METHOD_FFT) {
+        hipLaunchKernelGGL(k_fft_plane<T>, grid, dim3(FNTHR), fft_lds_bytes<T>(a.L), s, a);
+    } else if (reg_variant_ok(a.MB, a.KB)) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
         hipLaunchKernelGGL(k_lap_plane_reg<T>, grid, block, plane_reg_lds_bytes<T>(a.KB), s, a);
     } else {
@@ -535,7 +800,9 @@ hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
 template <typename T>
 hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s) {
     dim3 grid(grid_x, a.seg.n);
-    if (reg_variant_ok(a.MB, a.KB)) {
+    if (a.method == PSTD_METHOD_FFT) {
+        hipLaunchKernelGGL(k_fft_line<T>, grid, dim3(FNTHR), fft_lds_bytes<T>(a.L), s, a);
+    } else if (reg_variant_ok(a.MB, a.KB)) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
         hipLaunchKernelGGL(k_lap_line_reg<T>, grid, block, line_reg_lds_bytes<T>(a.KB), s, a);
     } else {
@@ -551,8 +818,8 @@ hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStre
     return hipGetLastError();
 }
 
-// Resident blocks per CU of the pass kernel that launch_plane / launch_line
-// would pick for (MB, KB), from the runtime's occupancy calculator.
+// Resident blocks per CU of the GEMM pass kernel that launch_plane /
+// launch_line would pick for (MB, KB), from the runtime's occupancy calculator.
 template <typename T> int pass_occupancy(int MB, int KB, bool line) {
     int n = 0;
     hipError_t e;
@@ -577,11 +844,19 @@ template <typename T> size_t pass_lds_bytes(int MB, int KB) {
     return std::max(plane_lds_bytes<T>(MB, KB), line_lds_bytes<T>(MB, KB));
 }
 
+bool gemm_variant_fits(int MB, int KB, size_t esz, size_t max_lds) {
+    if (MB > 16) return false;
+    const size_t lds = (esz == 8) ? pass_lds_bytes<double>(MB, KB) : pass_lds_bytes<float>(MB, KB);
+    return lds <= max_lds;
+}
+
 hipError_t configure_kernels(int max_lds) {
     const void *k[] = {(const void *)k_lap_plane_reg<double>, (const void *)k_lap_plane_reg<float>,
                        (const void *)k_lap_line_reg<double>,  (const void *)k_lap_line_reg<float>,
                        (const void *)k_lap_plane_lds<double>, (const void *)k_lap_plane_lds<float>,
-                       (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>};
+                       (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>,
+                       (const void *)k_fft_plane<double>,     (const void *)k_fft_plane<float>,
+                       (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>};
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return e;
@@ -597,10 +872,11 @@ template hipError_t launch_surface<double>(const double *, const SurfArgs &, lon
 template hipError_t launch_surface<float>(const float *, const SurfArgs &, long, hipStream_t);
 template size_t pass_lds_bytes<double>(int, int);
 template size_t pass_lds_bytes<float>(int, int);
-
-}  // namespace pstd
-
-namespace pstd {
 template int pass_occupancy<double>(int, int, bool);
 template int pass_occupancy<float>(int, int, bool);
+template size_t fft_lds_bytes<double>(int);
+template size_t fft_lds_bytes<float>(int);
+template int fft_occupancy<double>(int, bool);
+template int fft_occupancy<float>(int, bool);
+
 }  // namespace pstd

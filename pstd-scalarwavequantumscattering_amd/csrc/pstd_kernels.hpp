@@ -17,6 +17,22 @@ struct SegTable {
     int opoff[PSTD_MAX_SEG];    // operator fragment offset (elements)
 };
 
+// Stockham FFT plan of one axis for the FFT variant of the passes: radix
+// stages (2, 3, 4, 5, 7), twiddles e^{-2 pi i q k / (Ns P)} per stage, the
+// spectral multiplier alpha kappa^2 / L (forward unscaled, backward 1/L as
+// in DFTI, PSTD_QScat.cpp:502-505) and the window of every segment.
+#define PSTD_FFT_MAXST 16
+template <typename T>
+struct FftPlan {
+    int nst;
+    int rad[PSTD_FFT_MAXST];
+    int ns[PSTD_FFT_MAXST];
+    int twoff[PSTD_FFT_MAXST];   // complex offset into tw
+    const T *tw;                 // interleaved complex
+    const T *kmul;               // length L
+    const T *win;                // [segments][L]
+};
+
 // x and y passes: rows l of a segment are planes/rows with a common
 // contiguous column index.  psi0[core rows] += -i * (A psi_seg).
 template <typename T>
@@ -31,6 +47,9 @@ struct PlaneArgs {
     int col_tiles;
     int L, KB, MB;
     SegTable seg;
+    int method;          // PSTD_METHOD_GEMM or PSTD_METHOD_FFT
+    int fcol_tiles;      // column tiles of the FFT variant
+    FftPlan<T> fft;
 };
 
 // z pass fused with the pointwise terms of Update (PSTD_QScat.cpp:1268-1435).
@@ -51,10 +70,16 @@ struct LineArgs {
     const double *tfP[3];
     const double2 *tfA[3];
     const double2 *tfE[3];
+    // source region: lo..hi is the union of the two slab ranges and the TF
+    // box on an axis (min(ksi1a, ksi2a) .. max(ksi1b, ksi2b)); the source
+    // vanishes strictly inside (ksi1b, ksi2a) on all three axes
     int k1a[3], k1b[3], k2a[3], k2b[3];
     double sin_t, cos_t;
     // ABC
     const T *g[3];
+    int method;          // PSTD_METHOD_GEMM or PSTD_METHOD_FFT
+    int fline_tiles;     // line tiles of the FFT variant
+    FftPlan<T> fft;
 };
 
 struct SurfFace {
@@ -76,6 +101,11 @@ template <typename T> hipError_t launch_line(const LineArgs<T> &a, int grid_x, h
 template <typename T> hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStream_t s);
 template <typename T> size_t pass_lds_bytes(int MB, int KB);
 template <typename T> int pass_occupancy(int MB, int KB, bool line);
+template <typename T> size_t fft_lds_bytes(int L);
+template <typename T> int fft_occupancy(int L, bool line);
+bool gemm_variant_fits(int MB, int KB, size_t esz, size_t max_lds);
+constexpr int FFT_NC = 16;       // lines per FFT tile
+constexpr int FFT_LMAX = 512;    // longest local line of the FFT variant
 hipError_t configure_kernels(int max_lds);
 
 }  // namespace pstd

@@ -45,7 +45,14 @@ struct pstd_solver {
     int device = 0;
     int cus = 256;
     size_t max_lds = 64 * 1024;
-    int occ[3] = {1, 1, 1};       // resident blocks per CU of each axis pass
+    int occ[3] = {1, 1, 1};       // resident blocks per CU, GEMM pass of each axis
+    int focc[3] = {1, 1, 1};      // resident blocks per CU, FFT pass of each axis
+    int method[3] = {PSTD_METHOD_GEMM, PSTD_METHOD_GEMM, PSTD_METHOD_GEMM};
+    bool gemm_ok[3] = {false, false, false}, fft_ok[3] = {false, false, false};
+    FftTables fft[3];
+    void *fft_tw[3] = {nullptr, nullptr, nullptr};
+    void *fft_kmul[3] = {nullptr, nullptr, nullptr};
+    void *fft_win[3] = {nullptr, nullptr, nullptr};
     hipStream_t stream = nullptr;
     size_t esz = 8;               // sizeof(T)
     void *psi[2] = {nullptr, nullptr};
@@ -145,27 +152,41 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
     }
     s->esz = (p->precision == PSTD_FP32) ? 4 : 8;
     if ((rc = select_device(s))) { delete s; return rc; }
-    for (int a = 0; a < 3; ++a) {
-        s->lay[a] = op_layout(g.ax[a]);
-        size_t lds = (s->esz == 8) ? pass_lds_bytes<double>(s->lay[a].MB, s->lay[a].KB)
-                                   : pass_lds_bytes<float>(s->lay[a].MB, s->lay[a].KB);
-        if (lds > s->max_lds || s->lay[a].MB > 16) {
-            delete s;
-            return fail(PSTD_EINVAL, "axis " + std::to_string(a) + ": local length L=" + std::to_string(g.ax[a].L) +
-                                         " with core " + std::to_string(16 * s->lay[a].MB) +
-                                         " exceeds the on-chip operator tile of this precision");
-        }
-        const bool line = (a == 2);
-        s->occ[a] = (s->esz == 8) ? pass_occupancy<double>(s->lay[a].MB, s->lay[a].KB, line)
-                                  : pass_occupancy<float>(s->lay[a].MB, s->lay[a].KB, line);
-    }
-
-    // operators, fragments for MFMA
     const long double alpha = 2.0L * (long double)std::sin(p->dtau);   // PSTD_QScat.cpp:134
     for (int a = 0; a < 3; ++a) {
-        std::vector<long double> frag;
-        build_laplacian_fragments(g.ax[a], alpha, s->lay[a], frag);
-        if ((rc = upload_real_ld(s, s->op[a], frag, "operator"))) { delete s; return rc; }
+        const bool line = (a == 2);
+        s->lay[a] = op_layout(g.ax[a]);
+        s->gemm_ok[a] = gemm_variant_fits(s->lay[a].MB, s->lay[a].KB, s->esz, s->max_lds);
+        if (s->gemm_ok[a])
+            s->occ[a] = (s->esz == 8) ? pass_occupancy<double>(s->lay[a].MB, s->lay[a].KB, line)
+                                      : pass_occupancy<float>(s->lay[a].MB, s->lay[a].KB, line);
+        s->fft[a] = build_fft_tables(g.ax[a], alpha);
+        const size_t flds = (s->esz == 8) ? fft_lds_bytes<double>((int)g.ax[a].L) : fft_lds_bytes<float>((int)g.ax[a].L);
+        s->fft_ok[a] = s->fft[a].ok && g.ax[a].L <= FFT_LMAX && flds <= s->max_lds;
+        if (s->fft_ok[a])
+            s->focc[a] = (s->esz == 8) ? fft_occupancy<double>((int)g.ax[a].L, line)
+                                       : fft_occupancy<float>((int)g.ax[a].L, line);
+        if (!s->gemm_ok[a] && !s->fft_ok[a]) {
+            delete s;
+            return fail(PSTD_EINVAL, "axis " + std::to_string(a) + ": local length L=" + std::to_string(g.ax[a].L) +
+                                         " fits neither the on-chip operator tile nor the FFT variant "
+                                         "(L = 2^a 3^b 5^c 7^d <= 512)");
+        }
+        s->method[a] = s->gemm_ok[a] ? PSTD_METHOD_GEMM : PSTD_METHOD_FFT;
+    }
+
+    // operators: MFMA fragments and / or FFT tables
+    for (int a = 0; a < 3; ++a) {
+        if (s->gemm_ok[a]) {
+            std::vector<long double> frag;
+            build_laplacian_fragments(g.ax[a], alpha, s->lay[a], frag);
+            if ((rc = upload_real_ld(s, s->op[a], frag, "operator"))) { delete s; return rc; }
+        }
+        if (s->fft_ok[a]) {
+            if ((rc = upload_real_ld(s, s->fft_tw[a], s->fft[a].tw, "fft twiddles"))) { delete s; return rc; }
+            if ((rc = upload_real_ld(s, s->fft_kmul[a], s->fft[a].kmul, "fft multiplier"))) { delete s; return rc; }
+            if ((rc = upload_real_ld(s, s->fft_win[a], s->fft[a].win, "fft window"))) { delete s; return rc; }
+        }
         SegTable &T = s->seg[a];
         std::memset(&T, 0, sizeof(T));
         T.n = (int)g.ax[a].P;
@@ -241,71 +262,98 @@ int tmark(pstd_solver *s, int kind) {
         if (rc_) return rc_;        \
     } while (0)
 
-// Persistent grid width per segment: about four blocks per CU in total, a
-// multiple of 8 so that block (b, r) and (b, r+1) -- the two segments that
-// share an overlap of the same columns -- land on the same XCD (blocks are
-// dealt to the 8 XCDs round robin) and meet in its L2.
-// The width covers exactly the resident blocks (occupancy x CUs) so that no
-// partial second round of blocks is left.
+// Persistent grid width per segment: the resident blocks of the chosen
+// kernel (occupancy x CUs) divided among the segments, so that no partial
+// second round of blocks is left, rounded to a multiple of 8 so that block
+// (b, r) and (b, r+1) -- the two segments that share an overlap of the same
+// columns -- land on the same XCD (blocks are dealt to the 8 XCDs round
+// robin) and meet in its L2.
 int grid_for(pstd_solver *s, int axis, long tiles, int nseg) {
-    long target = (long)s->cus * s->occ[axis] / std::max(1, nseg);
+    const int occ = (s->method[axis] == PSTD_METHOD_FFT) ? s->focc[axis] : s->occ[axis];
+    long target = (long)s->cus * occ / std::max(1, nseg);
     target = std::max(8L, target / 8 * 8);
     return (int)std::max(1L, std::min(tiles, target));
 }
 
 template <typename T>
-int step_impl(pstd_solver *s) {
+FftPlan<T> fft_plan(pstd_solver *s, int a) {
+    FftPlan<T> f;
+    std::memset(&f, 0, sizeof(f));
+    if (!s->fft_ok[a]) return f;
+    f.nst = s->fft[a].nst;
+    for (int i = 0; i < f.nst; ++i) {
+        f.rad[i] = s->fft[a].rad[i];
+        f.ns[i] = s->fft[a].ns[i];
+        f.twoff[i] = s->fft[a].twoff[i];
+    }
+    f.tw = reinterpret_cast<const T *>(s->fft_tw[a]);
+    f.kmul = reinterpret_cast<const T *>(s->fft_kmul[a]);
+    f.win = reinterpret_cast<const T *>(s->fft_win[a]);
+    return f;
+}
+
+// Launch one axis pass (x: 0, y: 1 as plane passes; z: 2 fused) from psi
+// into psi0 at time index ntau.
+template <typename T>
+int launch_pass(pstd_solver *s, int axis, const T *psi, T *psi0, long ntau) {
     const Geometry &g = s->g;
     const long Nx = g.ax[0].N, Ny = g.ax[1].N, Nz = g.ax[2].N;
+    if (axis < 2) {
+        PlaneArgs<T> a;
+        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[axis]);
+        if (axis == 0) {   // rows = x planes, columns = (y, z)
+            a.row_stride = Ny * Nz; a.outer_stride = 0; a.outer_count = 1; a.ncols = (int)(Ny * Nz);
+        } else {           // rows = y, columns = z, outer = x
+            a.row_stride = Nz; a.outer_stride = Ny * Nz; a.outer_count = (int)Nx; a.ncols = (int)Nz;
+        }
+        a.col_tiles = (a.ncols + 31) / 32;
+        a.fcol_tiles = (a.ncols + FFT_NC - 1) / FFT_NC;
+        a.L = (int)g.ax[axis].L; a.KB = s->lay[axis].KB; a.MB = s->lay[axis].MB; a.seg = s->seg[axis];
+        a.method = s->method[axis];
+        a.fft = fft_plan<T>(s, axis);
+        const long tiles = (long)a.outer_count * (a.method == PSTD_METHOD_FFT ? a.fcol_tiles : a.col_tiles);
+        TMARK(s, axis);
+        HIPCHK(launch_plane<T>(a, grid_for(s, axis, tiles, a.seg.n), s->stream), axis == 0 ? "x pass" : "y pass");
+        TMARK(s, axis);
+        return 0;
+    }
+    LineArgs<T> a;
+    a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[2]);
+    a.nlines = Nx * Ny; a.Ny = (int)Ny; a.Nz = (int)Nz;
+    a.L = (int)g.ax[2].L; a.KB = s->lay[2].KB; a.MB = s->lay[2].MB;
+    a.line_tiles = (int)((a.nlines + 31) / 32);
+    a.fline_tiles = (int)((a.nlines + FFT_NC - 1) / FFT_NC);
+    a.seg = s->seg[2];
+    a.method = s->method[2];
+    a.fft = fft_plan<T>(s, 2);
+    a.vbox = reinterpret_cast<const T *>(s->vbox);
+    for (int d = 0; d < 3; ++d) {
+        a.vb1[d] = (int)g.vbox1[d]; a.vb2[d] = (int)g.vbox2[d];
+        a.tfP[d] = s->tfP[d]; a.tfA[d] = s->tfA[d]; a.tfE[d] = s->tfE[d];
+        a.k1a[d] = (int)std::min(g.ksi1a[d], g.ksi2a[d]); a.k1b[d] = (int)g.ksi1b[d];
+        a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)std::max(g.ksi1b[d], g.ksi2b[d]);
+        a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
+    }
+    const double t = (double)ntau * s->p.dtau;          // PSTD_QScat.cpp:1235
+    const double f = t / TWOPI;                         // :1236
+    a.vfac = (T)((f < 0.997) ? f : 1.0);                 // :1272
+    a.use_v = (s->has_v && s->nvbox > 0) ? 1 : 0;
+    a.sin_t = std::sin(t);                              // :1238-1239
+    a.cos_t = std::cos(t);
+    const long tiles = (a.method == PSTD_METHOD_FFT) ? a.fline_tiles : a.line_tiles;
+    TMARK(s, 2);
+    HIPCHK(launch_line<T>(a, grid_for(s, 2, tiles, a.seg.n), s->stream), "z pass");
+    TMARK(s, 2);
+    return 0;
+}
+
+template <typename T>
+int step_impl(pstd_solver *s) {
     const T *psi = reinterpret_cast<const T *>(s->psi[s->cur]);
     T *psi0 = reinterpret_cast<T *>(s->psi[1 - s->cur]);
-    // x pass: rows = x planes, columns = (y, z)
-    {
-        PlaneArgs<T> a;
-        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[0]);
-        a.row_stride = Ny * Nz; a.outer_stride = 0; a.outer_count = 1;
-        a.ncols = (int)(Ny * Nz); a.col_tiles = (a.ncols + 31) / 32;
-        a.L = (int)g.ax[0].L; a.KB = s->lay[0].KB; a.MB = s->lay[0].MB; a.seg = s->seg[0];
-        TMARK(s, 0);
-        HIPCHK(launch_plane<T>(a, grid_for(s, 0, (long)a.col_tiles, a.seg.n), s->stream), "x pass");
-        TMARK(s, 0);
-    }
-    // y pass: rows = y, columns = z, outer = x
-    {
-        PlaneArgs<T> a;
-        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[1]);
-        a.row_stride = Nz; a.outer_stride = Ny * Nz; a.outer_count = (int)Nx;
-        a.ncols = (int)Nz; a.col_tiles = (a.ncols + 31) / 32;
-        a.L = (int)g.ax[1].L; a.KB = s->lay[1].KB; a.MB = s->lay[1].MB; a.seg = s->seg[1];
-        TMARK(s, 1);
-        HIPCHK(launch_plane<T>(a, grid_for(s, 1, (long)a.col_tiles * Nx, a.seg.n), s->stream), "y pass");
-        TMARK(s, 1);
-    }
-    // z pass + potential + TF/SF + ABC
-    {
-        LineArgs<T> a;
-        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[2]);
-        a.nlines = Nx * Ny; a.Ny = (int)Ny; a.Nz = (int)Nz;
-        a.L = (int)g.ax[2].L; a.KB = s->lay[2].KB; a.MB = s->lay[2].MB;
-        a.line_tiles = (int)((a.nlines + 31) / 32);
-        a.seg = s->seg[2];
-        a.vbox = reinterpret_cast<const T *>(s->vbox);
-        for (int d = 0; d < 3; ++d) {
-            a.vb1[d] = (int)g.vbox1[d]; a.vb2[d] = (int)g.vbox2[d];
-            a.tfP[d] = s->tfP[d]; a.tfA[d] = s->tfA[d]; a.tfE[d] = s->tfE[d];
-            a.k1a[d] = (int)g.ksi1a[d]; a.k1b[d] = (int)g.ksi1b[d];
-            a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)g.ksi2b[d];
-            a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
-        }
-        const double t = (double)s->nTau * s->p.dtau;        // PSTD_QScat.cpp:1235
-        const double f = t / TWOPI;                         // :1236
-        a.vfac = (T)((f < 0.997) ? f : 1.0);                 // :1272
-        a.use_v = (s->has_v && s->nvbox > 0) ? 1 : 0;
-        a.sin_t = std::sin(t);                              // :1238-1239
-        a.cos_t = std::cos(t);
-        TMARK(s, 2);
-        HIPCHK(launch_line<T>(a, grid_for(s, 2, (long)a.line_tiles, a.seg.n), s->stream), "z pass");
-        TMARK(s, 2);
+    for (int axis = 0; axis < 3; ++axis) {
+        int rc = launch_pass<T>(s, axis, psi, psi0, s->nTau);
+        if (rc) return rc;
     }
     s->cur = 1 - s->cur;   // swap (:1441-1443)
     ++s->nTau;             // :1445
@@ -588,6 +636,22 @@ int pstd_line_operator(const pstd_params *p, int axis, int seg, int kind, double
         out[A.L + l] = op.fd[l];
     }
     return 0;
+}
+
+int pstd_set_method(pstd_solver *s, int axis, int method) {
+    if (!s || axis < 0 || axis > 2) return fail(PSTD_EINVAL, "bad solver or axis");
+    if (method == PSTD_METHOD_GEMM && !s->gemm_ok[axis])
+        return fail(PSTD_EINVAL, "GEMM tiles of this axis do not fit on chip");
+    if (method == PSTD_METHOD_FFT && !s->fft_ok[axis])
+        return fail(PSTD_EINVAL, "FFT variant needs L = 2^a 3^b 5^c 7^d <= 512");
+    if (method != PSTD_METHOD_GEMM && method != PSTD_METHOD_FFT) return fail(PSTD_EINVAL, "unknown method");
+    s->method[axis] = method;
+    return 0;
+}
+
+int pstd_get_method(const pstd_solver *s, int axis) {
+    if (!s || axis < 0 || axis > 2) return -1;
+    return s->method[axis];
 }
 
 int pstd_synchronize(pstd_solver *s) {

@@ -17,6 +17,8 @@ PSTD_ENODEV = 19
 
 PSTD_FP64 = 8
 PSTD_FP32 = 4
+PSTD_METHOD_GEMM = 0
+PSTD_METHOD_FFT = 1
 PSTD_DERIV_PS = 0
 PSTD_DERIV_FD = 1
 NWEIGHT = 8

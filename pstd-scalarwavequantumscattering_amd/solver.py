@@ -25,6 +25,7 @@ EXPORTS = [
     "pstd_time_index", "pstd_get_surfaces", "pstd_save_virtual_surfaces",
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
+    "pstd_set_method", "pstd_get_method",
 ]
 
 
@@ -79,6 +80,10 @@ def library():
         L.pstd_time_kernels.argtypes = [vp, l, l, dp]
         L.pstd_line_operator.restype = i
         L.pstd_line_operator.argtypes = [C.POINTER(PstdParams), i, i, i, dp, sz]
+        L.pstd_set_method.restype = i
+        L.pstd_set_method.argtypes = [vp, i, i]
+        L.pstd_get_method.restype = i
+        L.pstd_get_method.argtypes = [vp, i]
         _lib = L
     return _lib
 
@@ -182,6 +187,13 @@ class PSTDQScat:
         out = np.zeros(4)
         _check(library().pstd_time_kernels(self.h, int(nsteps), int(naccum), _dp(out)))
         return out
+
+    def set_method(self, axis, method):
+        """Line-operator method of an axis: 0 = GEMM (MFMA), 1 = FFT (LDS Stockham)."""
+        _check(library().pstd_set_method(self.h, int(axis), int(method)))
+
+    def methods(self):
+        return [library().pstd_get_method(self.h, a) for a in range(3)]
 
     def synchronize(self):
         _check(library().pstd_synchronize(self.h))

@@ -100,6 +100,56 @@ def test_fp32_against_fp64_oracle(cfg):
         assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 2e-5
 
 
+def _force(g, method):
+    """Select a line-operator method on every axis that offers it."""
+    used = []
+    for axis in range(3):
+        try:
+            g.set_method(axis, method)
+            used.append(axis)
+        except pstd_amd.PSTDError as e:
+            assert e.code == pstd_amd.PSTD_EINVAL
+    return used
+
+
+@pytest.mark.parametrize("cfg", CASES)
+def test_fft_method_random_state_one_step(cfg):
+    """The LDS Stockham FFT variant of every pass, one step from a random
+    state, against the oracle to 1e-12."""
+    p, V, o, g = _pair(cfg)
+    used = _force(g, pstd_amd.PSTD_METHOD_FFT)
+    assert g.methods() == [pstd_amd.PSTD_METHOD_FFT if a in used else pstd_amd.PSTD_METHOD_GEMM for a in range(3)]
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    b = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    for ntau in (9, 1450):
+        o.set_psi(0, a); o.set_psi(1, b); o.set_time_index(ntau)
+        g.set_psi(0, a); g.set_psi(1, b); g.set_time_index(ntau)
+        o.update(1)
+        g.Update(1)
+        assert rel(g.get_psi(0), o.get_psi(0)) < 1e-12
+
+
+LONG = dict(nprocs=(1, 2, 1), L=(48, 160, 40), h=15, n_abc=8, n_sca=9, n_ksi=8, theta=1.2, phi=0.9)
+
+
+@pytest.mark.parametrize("precision", [pstd_amd.PSTD_FP64, pstd_amd.PSTD_FP32])
+def test_long_line_fft_path(precision):
+    """L = 160 along y (h = 15, as in the example input) -- in FP64 too long
+    for the on-chip operator tiles, so the FFT variant is chosen."""
+    p, V, o, g = _pair(LONG, precision=precision)
+    if precision == pstd_amd.PSTD_FP64:
+        assert g.methods()[1] == pstd_amd.PSTD_METHOD_FFT
+    o.update(25)
+    g.Update(25)
+    tol = 1e-10 if precision == pstd_amd.PSTD_FP64 else 2e-5
+    assert rel(g.get_psi(0), o.get_psi(0)) < tol
+    o.update_accumulate(6)
+    g.update_accumulate(6)
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < tol
+
+
 def test_well96_fp64_parity():
     """The 96^3 square-well workload (2x1x1, L = 64/96/96): 150 steps, then a
     30-step accumulation, FP64 path against the oracle."""
