@@ -46,6 +46,7 @@ def gpu_run(pkg, W, precision, steps, warmup, fn):
     p, V = fn(precision=precision)
     s = pkg.PSTDQScat(p, V)
     npts = s.N[0] * s.N[1] * s.N[2]
+    tune = s.autotune(reps=10)          # GEMM vs FFT per axis (state untouched)
     s.Update(warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -54,7 +55,7 @@ def gpu_run(pkg, W, precision, steps, warmup, fn):
     dt = time.perf_counter() - t0
     acc = s.time_kernels(0, 3)          # surface capture, separately
     s.close()
-    return p, npts, dt, ms, acc
+    return p, npts, dt, ms, acc, tune
 
 
 def cpu_baseline(W, steps):
@@ -94,7 +95,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -106,7 +107,7 @@ def main():
     pkg = load_pkg()
     from pstd_amd import workloads as W
 
-    p, npts, dt, ms, acc = gpu_run(pkg, W, pkg.PSTD_FP64, args.steps, args.warmup, W.config_grf224)
+    p, npts, dt, ms, acc, tune = gpu_run(pkg, W, pkg.PSTD_FP64, args.steps, args.warmup, W.config_grf224)
     value = npts * args.steps / dt
     names = ["x_pass", "y_pass", "z_pass_fused"]
     dom = max(range(3), key=lambda k: ms[k])
@@ -134,15 +135,18 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(names[dom], "grf224_fp64")},
         "kernel_ms": {names[k]: ms[k] for k in range(3)} | {"surface_capture_per_axis": acc[3]},
+        "line_operator": {ax: ("fft" if tune["methods"][k] else "gemm") for k, ax in enumerate("xyz")},
+        "autotune_ms_gemm_fft": tune["ms"],
         "step_roofline_frac": bytes_per_pt * npts / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
     if not args.no_fp32:
         extra = {}
         for name, fn in (("grf224_fp32", W.config_grf224), ("wells320_fp32", W.config_wells320)):
-            _, n2, dt2, ms2, _ = gpu_run(pkg, W, pkg.PSTD_FP32, max(50, args.steps // 4), 10, fn)
+            _, n2, dt2, ms2, _, t2 = gpu_run(pkg, W, pkg.PSTD_FP32, max(50, args.steps // 4), 10, fn)
             extra[name] = {"value": n2 * max(50, args.steps // 4) / dt2,
                            "ms_per_step": dt2 / max(50, args.steps // 4) * 1e3,
-                           "kernel_ms": [float(v) for v in ms2[:3]]}
+                           "kernel_ms": [float(v) for v in ms2[:3]],
+                           "line_operator": ["fft" if m else "gemm" for m in t2["methods"]]}
         line["fp32_fp64phase"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)

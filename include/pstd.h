@@ -213,6 +213,17 @@ int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]
 int pstd_set_method(pstd_solver *s, int axis, int method);
 int pstd_get_method(const pstd_solver *s, int axis);
 
+/* Autotuner: for each axis, time every available method's pass (reps
+ * launches each, HIP events, on a scratch copy of psi^{n-1}: the solver
+ * state is not changed) and keep the faster.  With cache_path non-NULL, a
+ * JSON file of earlier choices keyed by (L, segments, halo, grid, precision,
+ * axis) is read first -- a cached axis is not timed -- and the file is
+ * rewritten with the new entries.  ms_out (may be NULL) receives
+ * [gemm_x, fft_x, gemm_y, fft_y, gemm_z, fft_z] in ms per launch (-1:
+ * unavailable, 0: taken from the cache).  Needs one extra grid-sized
+ * buffer while it runs. */
+int pstd_autotune(pstd_solver *s, const char *cache_path, int reps, double ms_out[6]);
+
 /* The dense real line operator the kernels apply for axis a and segment r:
  * kind 0: alpha * D2[core rows][0..L) * W (ncore x L, row-major), the
  *         windowed local-Fourier second derivative of UpdateX/Y/Z

@@ -654,6 +654,136 @@ int pstd_get_method(const pstd_solver *s, int axis) {
     return s->method[axis];
 }
 
+}  // extern "C"
+
+namespace {
+
+std::string tune_key(const pstd_solver *s, int axis) {
+    const Geometry &g = s->g;
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "L=%ld,P=%ld,h=%ld,N=%ldx%ldx%ld,prec=%d,axis=%d", g.ax[axis].L, g.ax[axis].P,
+                  g.ax[axis].h, g.ax[0].N, g.ax[1].N, g.ax[2].N, (int)s->esz, axis);
+    return buf;
+}
+
+struct TuneEntry {
+    std::string key;
+    int method;
+    double gemm_ms, fft_ms;
+};
+
+// The cache is written by this library, one entry per line:
+//   {"entries": {
+//    "<key>": {"method": "gemm", "gemm_ms": 0.1, "fft_ms": 0.2},
+//   ...}}
+std::vector<TuneEntry> read_cache(const char *path) {
+    std::vector<TuneEntry> out;
+    FILE *fp = path ? std::fopen(path, "r") : nullptr;
+    if (!fp) return out;
+    char line[1024];
+    while (std::fgets(line, sizeof(line), fp)) {
+        char key[512], meth[16];
+        double gm = -1, fm = -1;
+        if (std::sscanf(line, " \"%511[^\"]\": {\"method\": \"%15[a-z]\", \"gemm_ms\": %lf, \"fft_ms\": %lf", key, meth,
+                        &gm, &fm) == 4)
+            out.push_back({key, std::strcmp(meth, "fft") == 0 ? PSTD_METHOD_FFT : PSTD_METHOD_GEMM, gm, fm});
+    }
+    std::fclose(fp);
+    return out;
+}
+
+int write_cache(const char *path, const std::vector<TuneEntry> &e) {
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(PSTD_EIO, std::string("cannot write autotune cache ") + path);
+    std::fprintf(fp, "{\"entries\": {\n");
+    for (size_t i = 0; i < e.size(); ++i)
+        std::fprintf(fp, " \"%s\": {\"method\": \"%s\", \"gemm_ms\": %.6f, \"fft_ms\": %.6f}%s\n", e[i].key.c_str(),
+                     e[i].method == PSTD_METHOD_FFT ? "fft" : "gemm", e[i].gemm_ms, e[i].fft_ms,
+                     i + 1 < e.size() ? "," : "");
+    std::fprintf(fp, "}}\n");
+    return std::fclose(fp) == 0 ? 0 : fail(PSTD_EIO, "autotune cache write failed");
+}
+
+template <typename T>
+int time_pass(pstd_solver *s, int axis, int method, int reps, void *scratch, double &ms) {
+    const int saved = s->method[axis];
+    s->method[axis] = method;
+    const T *psi = reinterpret_cast<const T *>(s->psi[s->cur]);
+    T *out = reinterpret_cast<T *>(scratch);
+    int rc = launch_pass<T>(s, axis, psi, out, s->nTau);   // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && hipEventCreate(&e0) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventCreate");
+    if (!rc && hipEventCreate(&e1) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventCreate");
+    if (!rc && hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventRecord");
+    for (int i = 0; i < reps && !rc; ++i) rc = launch_pass<T>(s, axis, psi, out, s->nTau);
+    if (!rc && hipEventRecord(e1, s->stream) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventRecord");
+    float f = 0.f;
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventSynchronize");
+    if (!rc && hipEventElapsedTime(&f, e0, e1) != hipSuccess) rc = fail(PSTD_ENODEV, "hipEventElapsedTime");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    s->method[axis] = saved;
+    ms = f / std::max(1, reps);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pstd_autotune(pstd_solver *s, const char *cache_path, int reps, double ms_out[6]) {
+    if (!s || reps < 1) return fail(PSTD_EINVAL, "bad solver or reps");
+    std::vector<TuneEntry> cache = read_cache(cache_path);
+    double ms[6] = {-1, -1, -1, -1, -1, -1};
+    void *scratch = nullptr;
+    int rc = 0;
+    for (int axis = 0; axis < 3 && !rc; ++axis) {
+        const std::string key = tune_key(s, axis);
+        bool hit = false;
+        for (const TuneEntry &e : cache)
+            if (e.key == key && ((e.method == PSTD_METHOD_FFT && s->fft_ok[axis]) ||
+                                 (e.method == PSTD_METHOD_GEMM && s->gemm_ok[axis]))) {
+                s->method[axis] = e.method;
+                ms[2 * axis] = ms[2 * axis + 1] = 0.0;
+                hit = true;
+            }
+        if (hit) continue;
+        if (!scratch) {
+            if (hipMalloc(&scratch, (size_t)s->npts() * 2 * s->esz) != hipSuccess)
+                return fail(PSTD_ENOMEM, "autotune scratch");
+            if (hipMemcpyAsync(scratch, s->psi[1 - s->cur], (size_t)s->npts() * 2 * s->esz, hipMemcpyDeviceToDevice,
+                               s->stream) != hipSuccess)
+                rc = fail(PSTD_ENODEV, "autotune scratch copy");
+        }
+        for (int m = 0; m < 2 && !rc; ++m) {
+            const bool ok = (m == PSTD_METHOD_GEMM) ? s->gemm_ok[axis] : s->fft_ok[axis];
+            if (!ok) continue;
+            rc = (s->esz == 8) ? time_pass<double>(s, axis, m, reps, scratch, ms[2 * axis + m])
+                               : time_pass<float>(s, axis, m, reps, scratch, ms[2 * axis + m]);
+        }
+        if (rc) break;
+        int best = s->gemm_ok[axis] ? PSTD_METHOD_GEMM : PSTD_METHOD_FFT;
+        if (s->gemm_ok[axis] && s->fft_ok[axis] && ms[2 * axis + 1] < ms[2 * axis]) best = PSTD_METHOD_FFT;
+        s->method[axis] = best;
+        bool replaced = false;
+        for (TuneEntry &e : cache)
+            if (e.key == key) {
+                e = {key, best, ms[2 * axis], ms[2 * axis + 1]};
+                replaced = true;
+            }
+        if (!replaced) cache.push_back({key, best, ms[2 * axis], ms[2 * axis + 1]});
+    }
+    if (scratch) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(scratch);
+    }
+    if (rc) return rc;
+    if (ms_out)
+        for (int i = 0; i < 6; ++i) ms_out[i] = ms[i];
+    if (cache_path) return write_cache(cache_path, cache);
+    return 0;
+}
+
 int pstd_synchronize(pstd_solver *s) {
     if (!s) return fail(PSTD_EINVAL, "null solver");
     HIPCHK(hipStreamSynchronize(s->stream), "sync");

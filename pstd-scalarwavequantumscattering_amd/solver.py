@@ -25,7 +25,7 @@ EXPORTS = [
     "pstd_time_index", "pstd_get_surfaces", "pstd_save_virtual_surfaces",
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
-    "pstd_set_method", "pstd_get_method",
+    "pstd_set_method", "pstd_get_method", "pstd_autotune",
 ]
 
 
@@ -84,6 +84,8 @@ def library():
         L.pstd_set_method.argtypes = [vp, i, i]
         L.pstd_get_method.restype = i
         L.pstd_get_method.argtypes = [vp, i]
+        L.pstd_autotune.restype = i
+        L.pstd_autotune.argtypes = [vp, C.c_char_p, i, dp]
         _lib = L
     return _lib
 
@@ -194,6 +196,15 @@ class PSTDQScat:
 
     def methods(self):
         return [library().pstd_get_method(self.h, a) for a in range(3)]
+
+    def autotune(self, cache=None, reps=10):
+        """Time GEMM and FFT per axis, keep the faster (pstd_autotune).
+        Returns {"methods": [...], "ms": {"x": (gemm, fft), ...}}."""
+        ms = np.zeros(6)
+        path = str(cache).encode() if cache is not None else None
+        _check(library().pstd_autotune(self.h, path, int(reps), _dp(ms)))
+        return {"methods": self.methods(),
+                "ms": {ax: (float(ms[2 * k]), float(ms[2 * k + 1])) for k, ax in enumerate("xyz")}}
 
     def synchronize(self):
         _check(library().pstd_synchronize(self.h))

@@ -1,0 +1,59 @@
+"""GPU tests of the autotuner (pstd_autotune): it times both line-operator
+methods per axis without touching the solver state, keeps the faster, and
+persists / reuses its choices through a JSON cache."""
+import json
+
+import numpy as np
+import pytest
+
+import pstd_amd
+from pstd_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_autotune_choice_state_and_cache(tmp_path):
+    p, V = W.config_tiny(nprocs=(2, 2, 2), L=(32, 32, 32), h=10)
+    print("case autotune 44^3", flush=True)
+    g = pstd_amd.PSTDQScat(p, V)
+    g.Update(3)
+    before = (g.get_psi(0), g.get_psi(1), g.time_index())
+    cache = tmp_path / "tune.json"
+    res = g.autotune(cache=cache, reps=5)
+    for ax in "xyz":
+        gemm, fft = res["ms"][ax]
+        assert gemm > 0 and fft > 0
+    for k, ax in enumerate("xyz"):
+        gemm, fft = res["ms"][ax]
+        assert res["methods"][k] == (pstd_amd.PSTD_METHOD_FFT if fft < gemm else pstd_amd.PSTD_METHOD_GEMM)
+    after = (g.get_psi(0), g.get_psi(1), g.time_index())
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert before[2] == after[2]
+    # the tuned solver still agrees with the oracle
+    o = O.Oracle(p, V)
+    o.update(3)
+    o.update(10)
+    g.Update(10)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-10
+    # the cache holds one entry per axis and is reused without timing
+    d = json.loads(cache.read_text())
+    assert len(d["entries"]) == 3
+    g2 = pstd_amd.PSTDQScat(p, V)
+    res2 = g2.autotune(cache=cache, reps=5)
+    assert res2["methods"] == res["methods"]
+    assert all(v == (0.0, 0.0) for v in res2["ms"].values())
+
+
+def test_autotune_single_method_axis():
+    """An axis whose L has a prime factor 11 offers only GEMM: it is kept and
+    reported unavailable for FFT."""
+    p, V = W.config_tiny(nprocs=(1, 1, 1), L=(24, 20, 22), h=10)
+    print("case autotune L=22", flush=True)
+    g = pstd_amd.PSTDQScat(p, V)
+    res = g.autotune(reps=3)
+    assert res["ms"]["z"][1] == -1.0 and res["methods"][2] == pstd_amd.PSTD_METHOD_GEMM
