@@ -237,6 +237,31 @@ int pstd_line_operator(const pstd_params *p, int axis, int seg, int kind, double
 /* Last error message of this thread ("" if none). */
 const char *pstd_last_error(void);
 
+/* ---- batched independent solves -------------------------------------- */
+
+/* n independent problems (uneven grids, energies, incidences, potentials)
+ * stepped together: every pass is ONE kernel launch covering all problems'
+ * subdomains, and pstd_batch_update replays a captured HIP graph of two steps.
+ * No reference counterpart: the reference solves one problem per MPI job
+ * (qscat.cpp:156-198).  Each problem is created as by pstd_create_vbox
+ * (vboxes[i] with nvals[i] values, or V = 0 if vboxes is NULL); all must
+ * share one precision, and every axis must take the register GEMM variant
+ * (L <= 96, cores <= 128), else 22.  Problems start together at m_nTau = 1. */
+typedef struct pstd_batch pstd_batch;
+int pstd_batch_create(int n, const pstd_params *params, const double *const *vboxes, const size_t *nvals,
+                      pstd_batch **out);
+/* nsteps Updates of every problem (graph replays, asynchronous). */
+int pstd_batch_update(pstd_batch *b, long nsteps);
+/* nsteps x (Update + AccumulateSurfaceTerms) of every problem. */
+int pstd_batch_update_accumulate(pstd_batch *b, long nsteps);
+int pstd_batch_size(const pstd_batch *b);
+/* Borrowed handle of problem i for state access, surfaces and files
+ * (pstd_get_psi, pstd_get_surfaces, pstd_save_virtual_surfaces, ...); it
+ * must not be destroyed or stepped on its own. */
+pstd_solver *pstd_batch_solver(pstd_batch *b, int i);
+int pstd_batch_synchronize(pstd_batch *b);
+void pstd_batch_destroy(pstd_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,8 +117,15 @@ struct LineInfo {
 };
 static_assert(sizeof(LineInfo) <= LINEINFO_BYTES, "LineInfo must fit its LDS slot");
 
+// Time-dependent scalars of one step: sin t, cos t (PSTD_QScat.cpp:1238-1239)
+// and the potential ramp factor (:1272), FP64 from the host.
+template <typename T> struct Clock {
+    double sin_t, cos_t;
+    T vfac;
+};
+
 template <typename T>
-__device__ __forceinline__ void line_info(const LineArgs<T> &a, long qg, LineInfo &li) {
+__device__ __forceinline__ void line_info(const LineArgs<T> &a, const Clock<T> &clk, long qg, LineInfo &li) {
     if (qg >= a.nlines) {
         li.base = -1;
         return;
@@ -140,8 +147,8 @@ __device__ __forceinline__ void line_info(const LineArgs<T> &a, long qg, LineInf
         li.cxy.x = ax.x * py + ay.x * px;
         li.cxy.y = ax.y * py + ay.y * px;
         const double er = ex.x * ey.x - ex.y * ey.y, ei = ex.x * ey.y + ex.y * ey.x;
-        li.exy.x = er * a.sin_t - ei * a.cos_t;
-        li.exy.y = er * a.cos_t + ei * a.sin_t;
+        li.exy.x = er * clk.sin_t - ei * clk.cos_t;
+        li.exy.y = er * clk.cos_t + ei * clk.sin_t;
     }
 }
 
@@ -175,14 +182,15 @@ __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
 // One output point of the fused z pass: psi0 at dst, U = alpha D2 (W psi)
 // at the point (alpha folded), pn = psi^n at the point.
 template <typename T>
-__device__ __forceinline__ void z_point(const LineArgs<T> &a, const LineInfo &li, const ZInfo &zi,
-                                        typename VT<T>::c2 *dst, T ur, T ui, typename VT<T>::c2 pn) {
+__device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &clk, const LineInfo &li,
+                                        const ZInfo &zi, typename VT<T>::c2 *dst, T ur, T ui,
+                                        typename VT<T>::c2 pn) {
     typedef typename VT<T>::c2 C2;
     C2 acc = *dst;
     acc.x += ui;
     acc.y -= ur;
     if ((zi.flags & 1) && li.vrow >= 0) {
-        const T v = a.vfac * a.vbox[li.vrow + zi.zoff];
+        const T v = clk.vfac * a.vbox[li.vrow + zi.zoff];
         acc.x += v * pn.y;
         acc.y -= v * pn.x;
     }
@@ -201,7 +209,8 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const LineInfo &li
 }
 
 template <typename T>
-__device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const typename VT<T>::c2 *Ls, const LineInfo *LI,
+__device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<T> &clk, const typename VT<T>::c2 *Ls,
+                                              const LineInfo *LI,
                                               int LS, const ZInfo &zi, int z, int m, int i1, int kq,
                                               const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
                                               const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1) {
@@ -214,7 +223,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const typena
             const int line = hb * 16 + VT<T>::row(kq, q);
             const LineInfo &li = LI[line];
             if (li.base < 0) continue;
-            z_point<T>(a, li, zi, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
+            z_point<T>(a, clk, li, zi, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
                        Ls[line * LS + i1 + m]);
         }
     }
@@ -226,12 +235,11 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const typena
 // keeps its operator fragments in registers for the life of the block.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
+__device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     C2 *Bs = reinterpret_cast<C2 *>(smem);
-    const int r = blockIdx.y;
     const int KB = a.KB, Kp = 4 * KB;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -262,9 +270,9 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
             pre[j] = v;
         }
     };
-    int t = blockIdx.x;
+    int t = bx;
     if (t < ntiles) fetch(t);
-    for (; t < ntiles; t += gridDim.x) {
+    for (; t < ntiles; t += gx) {
         const int o = t / a.col_tiles;
         const long c0 = (long)(t - o * a.col_tiles) * PT;
         __syncthreads();
@@ -274,7 +282,7 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
             if (i < nel) Bs[i] = pre[j];
         }
         __syncthreads();
-        if (t + (int)gridDim.x < ntiles) fetch(t + gridDim.x);
+        if (t + gx < ntiles) fetch(t + gx);
         if (active) {
             ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
             const C2 *Bw = Bs + kq * PT + cq;
@@ -294,6 +302,11 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
+    plane_reg_body<T>(a, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // z pass (contiguous lines), register variant, with the pointwise terms of
 // Update fused into the epilogue.  The MFMA roles are swapped (psi is the A
@@ -301,11 +314,10 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
 // components of one output point and 16 lanes cover 16 consecutive z.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
+__device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<T> &clk, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int r = blockIdx.y;
     const int KB = a.KB, Kp = 4 * KB, LS = Kp + 1;
     C2 *Ls = reinterpret_cast<C2 *>(smem);
     LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)LT * LS * sizeof(C2)));
@@ -341,9 +353,9 @@ __global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
             pre[j] = v;
         }
     };
-    int t = blockIdx.x;
+    int t = bx;
     if (t < a.line_tiles) fetch(t);
-    for (; t < a.line_tiles; t += gridDim.x) {
+    for (; t < a.line_tiles; t += gx) {
         const long q0 = (long)t * LT;
         __syncthreads();
 #pragma unroll
@@ -354,9 +366,9 @@ __global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
                 Ls[line * LS + l] = pre[j];
             }
         }
-        if (tid < LT) line_info<T>(a, q0 + tid, LI[tid]);
+        if (tid < LT) line_info<T>(a, clk, q0 + tid, LI[tid]);
         __syncthreads();
-        if (t + (int)gridDim.x < a.line_tiles) fetch(t + gridDim.x);
+        if (t + gx < a.line_tiles) fetch(t + gx);
         if (!active) continue;
         ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
         const C2 *L0 = Ls + cq * LS + kq;
@@ -372,8 +384,40 @@ __global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
                 ai1 = mfma16(p1.y, af[kb], ai1);
             }
         }
-        if (lane_ok) line_epilogue<T>(a, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
+    const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
+    line_reg_body<T>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched passes: one launch covers every (problem, segment) of a batch of
+// independent solves.  blockIdx.y indexes a table of (problem, segment)
+// pairs; each problem's pass arguments live in device memory, and the
+// per-step time scalars of the z pass come from an FP64 table indexed by a
+// device step counter, so a captured graph replays any number of steps.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *list, const int2 *map) {
+    const int2 br = map[blockIdx.y];
+    plane_reg_body<T>(list[br.x], br.y, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *list, const int2 *map, const double *clock,
+                                                    const int *step, int nprob) {
+    const int2 br = map[blockIdx.y];
+    const double *c = clock + ((long)(*step) * nprob + br.x) * 3;
+    const Clock<T> clk = {c[1], c[2], (T)c[0]};
+    line_reg_body<T>(list[br.x], clk, br.y, blockIdx.x, gridDim.x);
+}
+
+__global__ void k_tick(int *step) {
+    if (threadIdx.x == 0) *step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -434,6 +478,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
     const int r = blockIdx.y;
     const int KB = a.KB, Kp = 4 * KB, LS = Kp + 1;
     T *As = reinterpret_cast<T *>(smem);
@@ -463,7 +508,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             if (l < a.L && q0 + line < a.nlines) v = psi[(q0 + line) * a.Nz + fft1 + l];
             Ls[line * LS + l] = v;
         }
-        if (tid < LT) line_info<T>(a, q0 + tid, LI[tid]);
+        if (tid < LT) line_info<T>(a, clk, q0 + tid, LI[tid]);
         __syncthreads();
         if (wave >= mbr) continue;
         ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
@@ -479,7 +524,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             ar1 = mfma16(p1.x, af, ar1);
             ai1 = mfma16(p1.y, af, ai1);
         }
-        if (lane_ok) line_epilogue<T>(a, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
 }
 
@@ -734,6 +779,7 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
     typedef typename VT<T>::c2 C2;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int LDW = FFT_NC + 1;
+    const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
     const int L = a.L, tid = threadIdx.x;
     C2 *S = reinterpret_cast<C2 *>(smem);
     LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)L * LDW * sizeof(C2)));
@@ -756,7 +802,7 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
             }
             S[l * LDW + c] = v;
         }
-        if (tid < FFT_NC) line_info<T>(a, q0 + tid, LI[tid]);
+        if (tid < FFT_NC) line_info<T>(a, clk, q0 + tid, LI[tid]);
         __syncthreads();
         fft_laplacian<T>(S, L, a.fft);
         for (int i = tid; i < ncore * FFT_NC; i += FNTHR) {
@@ -766,7 +812,7 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
             const int z = core1 + m;
             const ZInfo zi = z_info<T>(a, z);
             const C2 u = S[(i1 + m) * LDW + c];
-            z_point<T>(a, li, zi, psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
+            z_point<T>(a, clk, li, zi, psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
         }
     }
 }
@@ -818,6 +864,25 @@ hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStre
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_plane<T>, dim3(grid_x, nmap), dim3(block), plane_reg_lds_bytes<T>(KB), s, list, map);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
+                             int nprob, int grid_x, int block, int KB, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_line<T>, dim3(grid_x, nmap), dim3(block), line_reg_lds_bytes<T>(KB), s, list, map, clock,
+                       step, nprob);
+    return hipGetLastError();
+}
+hipError_t launch_tick(int *step, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, step);
+    return hipGetLastError();
+}
+bool batch_variant_ok(int MB, int KB) { return reg_variant_ok(MB, KB); }
+
 // Resident blocks per CU of the GEMM pass kernel that launch_plane /
 // launch_line would pick for (MB, KB), from the runtime's occupancy calculator.
 template <typename T> int pass_occupancy(int MB, int KB, bool line) {
@@ -856,7 +921,9 @@ hipError_t configure_kernels(int max_lds) {
                        (const void *)k_lap_plane_lds<double>, (const void *)k_lap_plane_lds<float>,
                        (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>,
                        (const void *)k_fft_plane<double>,     (const void *)k_fft_plane<float>,
-                       (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>};
+                       (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>,
+                       (const void *)k_batch_plane<double>,   (const void *)k_batch_plane<float>,
+                       (const void *)k_batch_line<double>,    (const void *)k_batch_line<float>};
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return e;
@@ -878,5 +945,11 @@ template size_t fft_lds_bytes<double>(int);
 template size_t fft_lds_bytes<float>(int);
 template int fft_occupancy<double>(int, bool);
 template int fft_occupancy<float>(int, bool);
+template hipError_t launch_batch_plane<double>(const PlaneArgs<double> *, const int2 *, int, int, int, int, hipStream_t);
+template hipError_t launch_batch_plane<float>(const PlaneArgs<float> *, const int2 *, int, int, int, int, hipStream_t);
+template hipError_t launch_batch_line<double>(const LineArgs<double> *, const int2 *, int, const double *, const int *,
+                                              int, int, int, int, hipStream_t);
+template hipError_t launch_batch_line<float>(const LineArgs<float> *, const int2 *, int, const double *, const int *,
+                                             int, int, int, int, hipStream_t);
 
 }  // namespace pstd

@@ -104,6 +104,14 @@ template <typename T> int pass_occupancy(int MB, int KB, bool line);
 template <typename T> size_t fft_lds_bytes(int L);
 template <typename T> int fft_occupancy(int L, bool line);
 bool gemm_variant_fits(int MB, int KB, size_t esz, size_t max_lds);
+bool batch_variant_ok(int MB, int KB);
+template <typename T>
+hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
+                             int nprob, int grid_x, int block, int KB, hipStream_t s);
+hipError_t launch_tick(int *step, hipStream_t s);
 constexpr int FFT_NC = 16;       // lines per FFT tile
 constexpr int FFT_LMAX = 512;    // longest local line of the FFT variant
 hipError_t configure_kernels(int max_lds);

@@ -77,9 +77,12 @@ struct pstd_solver {
     std::vector<hipEvent_t> tev;
     std::vector<int> tkind;
 
+    bool owns_stream = true;      // false when a batch shares its stream
+
     ~pstd_solver() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *q : allocs) (void)hipFree(q);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
     template <typename X>
     int alloc(X *&ptr, size_t bytes, const char *what) {
@@ -275,6 +278,18 @@ int grid_for(pstd_solver *s, int axis, long tiles, int nseg) {
     return (int)std::max(1L, std::min(tiles, target));
 }
 
+// Time scalars of the step that starts at time index ntau: the potential
+// ramp f = t / 2 pi below 0.997, else 1 (PSTD_QScat.cpp:1235-1236, :1272),
+// and sin t, cos t of the TF/SF time factor (:1238-1239); FP64.
+template <typename T>
+void step_clock(const pstd_solver *s, long ntau, T &vfac, double &sin_t, double &cos_t) {
+    const double t = (double)ntau * s->p.dtau;
+    const double f = t / TWOPI;
+    vfac = (T)((f < 0.997) ? f : 1.0);
+    sin_t = std::sin(t);
+    cos_t = std::cos(t);
+}
+
 template <typename T>
 FftPlan<T> fft_plan(pstd_solver *s, int a) {
     FftPlan<T> f;
@@ -292,32 +307,34 @@ FftPlan<T> fft_plan(pstd_solver *s, int a) {
     return f;
 }
 
-// Launch one axis pass (x: 0, y: 1 as plane passes; z: 2 fused) from psi
-// into psi0 at time index ntau.
+// Arguments of the x (axis 0) and y (axis 1) passes from psi into psi0.
 template <typename T>
-int launch_pass(pstd_solver *s, int axis, const T *psi, T *psi0, long ntau) {
+PlaneArgs<T> plane_args(pstd_solver *s, int axis, const T *psi, T *psi0) {
     const Geometry &g = s->g;
     const long Nx = g.ax[0].N, Ny = g.ax[1].N, Nz = g.ax[2].N;
-    if (axis < 2) {
-        PlaneArgs<T> a;
-        a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[axis]);
-        if (axis == 0) {   // rows = x planes, columns = (y, z)
-            a.row_stride = Ny * Nz; a.outer_stride = 0; a.outer_count = 1; a.ncols = (int)(Ny * Nz);
-        } else {           // rows = y, columns = z, outer = x
-            a.row_stride = Nz; a.outer_stride = Ny * Nz; a.outer_count = (int)Nx; a.ncols = (int)Nz;
-        }
-        a.col_tiles = (a.ncols + 31) / 32;
-        a.fcol_tiles = (a.ncols + FFT_NC - 1) / FFT_NC;
-        a.L = (int)g.ax[axis].L; a.KB = s->lay[axis].KB; a.MB = s->lay[axis].MB; a.seg = s->seg[axis];
-        a.method = s->method[axis];
-        a.fft = fft_plan<T>(s, axis);
-        const long tiles = (long)a.outer_count * (a.method == PSTD_METHOD_FFT ? a.fcol_tiles : a.col_tiles);
-        TMARK(s, axis);
-        HIPCHK(launch_plane<T>(a, grid_for(s, axis, tiles, a.seg.n), s->stream), axis == 0 ? "x pass" : "y pass");
-        TMARK(s, axis);
-        return 0;
+    PlaneArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[axis]);
+    if (axis == 0) {   // rows = x planes, columns = (y, z)
+        a.row_stride = Ny * Nz; a.outer_stride = 0; a.outer_count = 1; a.ncols = (int)(Ny * Nz);
+    } else {           // rows = y, columns = z, outer = x
+        a.row_stride = Nz; a.outer_stride = Ny * Nz; a.outer_count = (int)Nx; a.ncols = (int)Nz;
     }
+    a.col_tiles = (a.ncols + 31) / 32;
+    a.fcol_tiles = (a.ncols + FFT_NC - 1) / FFT_NC;
+    a.L = (int)g.ax[axis].L; a.KB = s->lay[axis].KB; a.MB = s->lay[axis].MB; a.seg = s->seg[axis];
+    a.method = s->method[axis];
+    a.fft = fft_plan<T>(s, axis);
+    return a;
+}
+
+// Arguments of the fused z pass from psi into psi0 at time index ntau.
+template <typename T>
+LineArgs<T> line_args(pstd_solver *s, const T *psi, T *psi0, long ntau) {
+    const Geometry &g = s->g;
+    const long Nx = g.ax[0].N, Ny = g.ax[1].N, Nz = g.ax[2].N;
     LineArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
     a.psi = psi; a.psi0 = psi0; a.op = reinterpret_cast<const T *>(s->op[2]);
     a.nlines = Nx * Ny; a.Ny = (int)Ny; a.Nz = (int)Nz;
     a.L = (int)g.ax[2].L; a.KB = s->lay[2].KB; a.MB = s->lay[2].MB;
@@ -334,12 +351,24 @@ int launch_pass(pstd_solver *s, int axis, const T *psi, T *psi0, long ntau) {
         a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)std::max(g.ksi1b[d], g.ksi2b[d]);
         a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
     }
-    const double t = (double)ntau * s->p.dtau;          // PSTD_QScat.cpp:1235
-    const double f = t / TWOPI;                         // :1236
-    a.vfac = (T)((f < 0.997) ? f : 1.0);                 // :1272
+    step_clock(s, ntau, a.vfac, a.sin_t, a.cos_t);
     a.use_v = (s->has_v && s->nvbox > 0) ? 1 : 0;
-    a.sin_t = std::sin(t);                              // :1238-1239
-    a.cos_t = std::cos(t);
+    return a;
+}
+
+// Launch one axis pass (x: 0, y: 1 as plane passes; z: 2 fused) from psi
+// into psi0 at time index ntau.
+template <typename T>
+int launch_pass(pstd_solver *s, int axis, const T *psi, T *psi0, long ntau) {
+    if (axis < 2) {
+        PlaneArgs<T> a = plane_args<T>(s, axis, psi, psi0);
+        const long tiles = (long)a.outer_count * (a.method == PSTD_METHOD_FFT ? a.fcol_tiles : a.col_tiles);
+        TMARK(s, axis);
+        HIPCHK(launch_plane<T>(a, grid_for(s, axis, tiles, a.seg.n), s->stream), axis == 0 ? "x pass" : "y pass");
+        TMARK(s, axis);
+        return 0;
+    }
+    LineArgs<T> a = line_args<T>(s, psi, psi0, ntau);
     const long tiles = (a.method == PSTD_METHOD_FFT) ? a.fline_tiles : a.line_tiles;
     TMARK(s, 2);
     HIPCHK(launch_line<T>(a, grid_for(s, 2, tiles, a.seg.n), s->stream), "z pass");
@@ -789,5 +818,249 @@ int pstd_synchronize(pstd_solver *s) {
     HIPCHK(hipStreamSynchronize(s->stream), "sync");
     return 0;
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched independent solves: one launch per pass for all problems, the
+// step loop captured as a graph of two steps (one psi swap cycle).
+// ===========================================================================
+struct pstd_batch {
+    std::vector<pstd_solver *> sv;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;
+    int KB = 0, block = 256;
+    void *lists[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [parity][pass]
+    int2 *maps[3] = {nullptr, nullptr, nullptr};
+    int nmap[3] = {0, 0, 0};
+    int gx[3] = {8, 8, 8};
+    double *clock = nullptr;      // [step][problem][vfac, sin t, cos t]
+    long clock_cap = 0;
+    int *step = nullptr;          // steps done since creation (device)
+    long steps_done = 0;
+    long ntau0 = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~pstd_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (int p = 0; p < 2; ++p)
+            for (int k = 0; k < 3; ++k)
+                if (lists[p][k]) (void)hipFree(lists[p][k]);
+        for (int k = 0; k < 3; ++k)
+            if (maps[k]) (void)hipFree(maps[k]);
+        if (clock) (void)hipFree(clock);
+        if (step) (void)hipFree(step);
+        for (pstd_solver *s : sv) delete s;
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int batch_fill_clock(pstd_batch *b, long upto) {
+    if (upto <= b->clock_cap) return 0;
+    long cap = std::max(upto, std::max(2 * b->clock_cap, 4096L));
+    const size_t nprob = b->sv.size();
+    std::vector<double> h((size_t)cap * nprob * 3);
+    for (long k = 0; k < cap; ++k)
+        for (size_t i = 0; i < nprob; ++i) {
+            double vf, st, ct;
+            step_clock<double>(b->sv[i], b->ntau0 + k, vf, st, ct);
+            double *e = &h[((size_t)k * nprob + i) * 3];
+            e[0] = vf; e[1] = st; e[2] = ct;
+        }
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    HIPCHK(hipStreamSynchronize(b->stream), "sync");
+    if (b->clock) (void)hipFree(b->clock);
+    b->clock = nullptr;
+    HIPCHK(hipMalloc(&b->clock, h.size() * sizeof(double)), "batch clock");
+    HIPCHK(hipMemcpy(b->clock, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "batch clock");
+    b->clock_cap = cap;
+    return 0;
+}
+
+template <typename T>
+int batch_build(pstd_batch *b) {
+    const int nprob = (int)b->sv.size();
+    std::vector<int2> maps[3];
+    long max_tiles[3] = {1, 1, 1};
+    for (int p = 0; p < 2; ++p) {
+        std::vector<PlaneArgs<T>> lx, ly;
+        std::vector<LineArgs<T>> lz;
+        for (int i = 0; i < nprob; ++i) {
+            pstd_solver *s = b->sv[i];
+            const T *psi = reinterpret_cast<const T *>(s->psi[p]);
+            T *psi0 = reinterpret_cast<T *>(s->psi[1 - p]);
+            lx.push_back(plane_args<T>(s, 0, psi, psi0));
+            ly.push_back(plane_args<T>(s, 1, psi, psi0));
+            lz.push_back(line_args<T>(s, psi, psi0, s->nTau));
+            if (p == 0) {
+                for (int axis = 0; axis < 3; ++axis)
+                    for (int r = 0; r < (int)s->g.ax[axis].P; ++r) maps[axis].push_back(make_int2(i, r));
+                max_tiles[0] = std::max(max_tiles[0], (long)lx.back().col_tiles);
+                max_tiles[1] = std::max(max_tiles[1], (long)ly.back().col_tiles * ly.back().outer_count);
+                max_tiles[2] = std::max(max_tiles[2], (long)lz.back().line_tiles);
+            }
+        }
+        HIPCHK(hipMalloc(&b->lists[p][0], lx.size() * sizeof(PlaneArgs<T>)), "batch args");
+        HIPCHK(hipMalloc(&b->lists[p][1], ly.size() * sizeof(PlaneArgs<T>)), "batch args");
+        HIPCHK(hipMalloc(&b->lists[p][2], lz.size() * sizeof(LineArgs<T>)), "batch args");
+        HIPCHK(hipMemcpy(b->lists[p][0], lx.data(), lx.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice), "batch args");
+        HIPCHK(hipMemcpy(b->lists[p][1], ly.data(), ly.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice), "batch args");
+        HIPCHK(hipMemcpy(b->lists[p][2], lz.data(), lz.size() * sizeof(LineArgs<T>), hipMemcpyHostToDevice), "batch args");
+    }
+    for (int k = 0; k < 3; ++k) {
+        b->nmap[k] = (int)maps[k].size();
+        HIPCHK(hipMalloc(&b->maps[k], maps[k].size() * sizeof(int2)), "batch map");
+        HIPCHK(hipMemcpy(b->maps[k], maps[k].data(), maps[k].size() * sizeof(int2), hipMemcpyHostToDevice), "batch map");
+        // about four tiles per block for the largest problem; smaller problems
+        // leave their surplus blocks idle, and the many blocks balance
+        // dynamically over the CUs
+        long g = (max_tiles[k] + 3) / 4;
+        g = std::max(8L, (g + 7) / 8 * 8);
+        b->gx[k] = (int)g;
+    }
+    return 0;
+}
+
+template <typename T>
+int batch_step_launch(pstd_batch *b, int parity) {
+    const int nprob = (int)b->sv.size();
+    HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][0]), b->maps[0], b->nmap[0],
+                                 b->gx[0], b->block, b->KB, b->stream), "batch x pass");
+    HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][1]), b->maps[1], b->nmap[1],
+                                 b->gx[1], b->block, b->KB, b->stream), "batch y pass");
+    HIPCHK(launch_batch_line<T>(reinterpret_cast<const LineArgs<T> *>(b->lists[parity][2]), b->maps[2], b->nmap[2],
+                                b->clock, b->step, nprob, b->gx[2], b->block, b->KB, b->stream), "batch z pass");
+    HIPCHK(launch_tick(b->step, b->stream), "batch tick");
+    return 0;
+}
+
+void batch_advance(pstd_batch *b, long n) {
+    b->steps_done += n;
+    for (pstd_solver *s : b->sv) {
+        if (n % 2) s->cur = 1 - s->cur;
+        s->nTau += n;
+    }
+}
+
+template <typename T>
+int batch_capture(pstd_batch *b) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    int rc = batch_step_launch<T>(b, 0);
+    if (!rc) rc = batch_step_launch<T>(b, 1);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hipfail(e, "graph capture");
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+    return 0;
+}
+
+template <typename T>
+int batch_update(pstd_batch *b, long n, bool accumulate) {
+    int rc = batch_fill_clock(b, b->steps_done + n + 2);
+    if (rc) return rc;
+    while (n > 0) {
+        const int parity = (int)(b->steps_done % 2);
+        if (!accumulate && parity == 0 && n >= 2) {
+            if (!b->exec && (rc = batch_capture<T>(b))) return rc;
+            HIPCHK(hipGraphLaunch(b->exec, b->stream), "graph launch");
+            batch_advance(b, 2);
+            n -= 2;
+            continue;
+        }
+        if ((rc = batch_step_launch<T>(b, parity))) return rc;
+        batch_advance(b, 1);
+        --n;
+        if (accumulate)
+            for (pstd_solver *s : b->sv)
+                if ((rc = accumulate_impl<T>(s))) return rc;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pstd_batch_create(int n, const pstd_params *params, const double *const *vboxes, const size_t *nvals,
+                      pstd_batch **out) {
+    if (n < 1 || !params || !out) return fail(PSTD_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    pstd_batch *b = new (std::nothrow) pstd_batch();
+    if (!b) return fail(PSTD_ENOMEM, "host allocation");
+    int rc = 0;
+    for (int i = 0; i < n && !rc; ++i) {
+        pstd_solver *s = nullptr;
+        const double *vb = vboxes ? vboxes[i] : nullptr;
+        rc = create_common(&params[i], vb, vb ? nvals[i] : 0, nullptr, &s);
+        if (rc) break;
+        b->sv.push_back(s);
+        if (s->esz != b->sv[0]->esz) rc = fail(PSTD_EINVAL, "all problems of a batch need the same precision");
+        for (int a = 0; a < 3 && !rc; ++a) {
+            if (!s->gemm_ok[a] || !batch_variant_ok(s->lay[a].MB, s->lay[a].KB))
+                rc = fail(PSTD_EINVAL, "batched passes need L <= 96 and cores <= 128 on every axis");
+            s->method[a] = PSTD_METHOD_GEMM;
+            b->KB = std::max(b->KB, s->lay[a].KB);
+            b->block = std::max(b->block, 64 * std::max(4, s->lay[a].MB));
+        }
+    }
+    if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(PSTD_ENODEV, "hipStreamCreate");
+    if (rc) { delete b; return rc; }
+    for (pstd_solver *s : b->sv) {   // one stream for the whole batch
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+        s->stream = b->stream;
+        s->owns_stream = false;
+    }
+    b->esz = b->sv[0]->esz;
+    b->ntau0 = b->sv[0]->nTau;
+    if (hipMalloc(&b->step, sizeof(int)) != hipSuccess) { delete b; return fail(PSTD_ENOMEM, "batch counter"); }
+    const int zero = 0;
+    if (hipMemcpy(b->step, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        delete b;
+        return fail(PSTD_ENODEV, "batch counter");
+    }
+    rc = (b->esz == 8) ? batch_build<double>(b) : batch_build<float>(b);
+    if (!rc) rc = batch_fill_clock(b, 4096);
+    if (rc) { delete b; return rc; }
+    *out = b;
+    return 0;
+}
+
+int pstd_batch_update(pstd_batch *b, long nsteps) {
+    if (!b || nsteps < 0) return fail(PSTD_EINVAL, "bad batch or step count");
+    return (b->esz == 8) ? batch_update<double>(b, nsteps, false) : batch_update<float>(b, nsteps, false);
+}
+
+int pstd_batch_update_accumulate(pstd_batch *b, long nsteps) {
+    if (!b || nsteps < 0) return fail(PSTD_EINVAL, "bad batch or step count");
+    return (b->esz == 8) ? batch_update<double>(b, nsteps, true) : batch_update<float>(b, nsteps, true);
+}
+
+int pstd_batch_size(const pstd_batch *b) { return b ? (int)b->sv.size() : -1; }
+
+pstd_solver *pstd_batch_solver(pstd_batch *b, int i) {
+    if (!b || i < 0 || i >= (int)b->sv.size()) return nullptr;
+    return b->sv[i];
+}
+
+int pstd_batch_synchronize(pstd_batch *b) {
+    if (!b) return fail(PSTD_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream), "sync");
+    return 0;
+}
+
+void pstd_batch_destroy(pstd_batch *b) { delete b; }
 
 }  // extern "C"

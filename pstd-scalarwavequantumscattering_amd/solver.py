@@ -26,6 +26,8 @@ EXPORTS = [
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune",
+    "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
+    "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
 ]
 
 
@@ -86,6 +88,19 @@ def library():
         L.pstd_get_method.argtypes = [vp, i]
         L.pstd_autotune.restype = i
         L.pstd_autotune.argtypes = [vp, C.c_char_p, i, dp]
+        L.pstd_batch_create.restype = i
+        L.pstd_batch_create.argtypes = [i, C.POINTER(PstdParams), C.POINTER(dp), C.POINTER(sz), C.POINTER(vp)]
+        for name in ("pstd_batch_update", "pstd_batch_update_accumulate"):
+            getattr(L, name).restype = i
+            getattr(L, name).argtypes = [vp, l]
+        L.pstd_batch_size.restype = i
+        L.pstd_batch_size.argtypes = [vp]
+        L.pstd_batch_solver.restype = vp
+        L.pstd_batch_solver.argtypes = [vp, i]
+        L.pstd_batch_synchronize.restype = i
+        L.pstd_batch_synchronize.argtypes = [vp]
+        L.pstd_batch_destroy.restype = None
+        L.pstd_batch_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -238,5 +253,73 @@ class PSTDQScat:
         _check(library().pstd_set_psi(self.h, int(level), _dp(a), a.size))
 
 
-__all__ = ["PSTDQScat", "line_operator", "PSTDError", "load_input", "compute_geometry", "library",
+class _BatchMember(PSTDQScat):
+    """Borrowed view of one problem of a batch (no Init, no destroy)."""
+
+    def __init__(self, handle, params, batch):
+        self.params = params
+        self.h = C.c_void_p(handle)
+        self._batch = batch
+        g = compute_geometry(params)
+        self.geometry = g
+        self.N = tuple(g.N)
+
+    def close(self):
+        self.h = C.c_void_p()
+
+
+class PSTDBatch:
+    """Independent problems stepped together (pstd_batch_*): one kernel
+    launch per pass for all problems, steps replayed from a captured graph."""
+
+    def __init__(self, params_list, vboxes=None):
+        n = len(params_list)
+        arr = (PstdParams * n)(*params_list)
+        self._vb = []
+        if vboxes is not None:
+            ptrs = (C.POINTER(C.c_double) * n)()
+            sizes = (C.c_size_t * n)()
+            for i, v in enumerate(vboxes):
+                a = np.ascontiguousarray(v, dtype=np.float64)
+                self._vb.append(a)
+                ptrs[i] = _dp(a)
+                sizes[i] = a.size
+        else:
+            ptrs, sizes = None, None
+        h = C.c_void_p()
+        _check(library().pstd_batch_create(n, arr, ptrs, sizes, C.byref(h)))
+        self.h = h
+        self.params = list(params_list)
+        self.members = [_BatchMember(library().pstd_batch_solver(h, i), params_list[i], self) for i in range(n)]
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, i):
+        return self.members[i]
+
+    def Update(self, nsteps=1):
+        _check(library().pstd_batch_update(self.h, int(nsteps)))
+
+    def update_accumulate(self, nsteps):
+        _check(library().pstd_batch_update_accumulate(self.h, int(nsteps)))
+
+    def synchronize(self):
+        _check(library().pstd_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            for m in self.members:
+                m.close()
+            library().pstd_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["PSTDQScat", "PSTDBatch", "line_operator", "PSTDError", "load_input", "compute_geometry", "library",
            "LIB_PATH", "EXPORTS", "PSTD_DERIV_PS", "PSTD_DERIV_FD", "PSTD_FP64"]

@@ -203,6 +203,40 @@ def config_wells320(steps=8000, precision=PSTD_FP32, seed=3):
     return p, gaussian_wells(p, k, 20, 15.0, 0.5, 2.0, -3.0, 3.0, seed + 1000)
 
 
+def batch_problem(p_index, N, k, precision=PSTD_FP32, steps=4000, dx=0.25, seed_base=4000):
+    """One problem of the batched workload: N^3 with L = 64, h = 16 (N = 64,
+    96, 128, 160 for 1..4 subdomains per axis), 8-cell absorber, n_sca = 5,
+    n_ksi = 6, energy k, seeded incidence; even problems a spherical well of
+    depth -1, odd problems a Gaussian random field (sigma 0.5, correlation
+    length 1), both cut off at 0.35 x box, clamped to the potential box."""
+    P = (N - 32) // 32
+    rng = SplitMix64(seed_base + p_index)
+    th, ph = random_direction(rng)
+    dxr = k * dx
+    p = make_params((P, P, P), stable_dtau(dxr), (dxr,) * 3, (64, 64, 64), 8, 5, 6, 16,
+                    th, ph, k * k / 2, 5.0, 4.0 / 8, steps, f"batch{p_index}", precision)
+    lo, hi = vbox_range(p)
+    half = min(hi[a] - lo[a] for a in range(3)) / 2.0 * dx
+    rc = min(0.35 * N * dx, half)
+    if p_index % 2 == 0:
+        V = square_well(p, k, rc, -1.0)
+    else:
+        V = gaussian_random_field(p, k, dx, 0.5, 1.0, rc, seed_base + p_index)
+    return p, V
+
+
+def config_batch48(steps=4000, precision=PSTD_FP32, n=48, seed=4000):
+    """48 independent problems: grids drawn from {64, 96, 128, 160}^3 with
+    dx = 0.25, k ~ U[1.0, 3.5], seeded incidence, wells / random fields."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(n):
+        N = (64, 96, 128, 160)[int(rng.uniform() * 4) % 4]
+        k = 1.0 + 2.5 * rng.uniform()
+        out.append(batch_problem(i, N, k, precision, steps, seed_base=seed))
+    return out
+
+
 def config_example():
     """InputExample.txt: 1x2x1, dtau = pi/1000, dx = 2 pi/20, L = 288 160 288,
     40 41 12 15, theta = phi = pi/2, E0 = 0.327216805, U0 = 5, alpha = 0.1,
