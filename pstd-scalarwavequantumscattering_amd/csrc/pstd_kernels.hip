@@ -50,6 +50,34 @@ constexpr int LT = 32;      // lines per z tile (two 16-line blocks)
 constexpr int KBMAX = 24;   // register variant: L <= 96
 constexpr int NLD = (4 * KBMAX * PT + 255) / 256;   // tile loads per thread (>= 256 threads)
 
+// Pointers read from a device-side argument table (the batched passes) are
+// generic to the compiler, which then emits flat memory instructions; those
+// count on the LDS counter too, so every LDS wait would also wait for the
+// tile prefetch.  All field arrays live in global memory: glb() says so, and
+// ldv/stv move complex pairs through such pointers as native 2-vectors.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <typename P> __device__ __forceinline__ GLOBAL_AS P *glb(P *p) { return (GLOBAL_AS P *)p; }
+typedef double nat_d2 __attribute__((ext_vector_type(2)));
+typedef float nat_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ldv(const double2 *p) { return *p; }
+__device__ __forceinline__ float2 ldv(const float2 *p) { return *p; }
+__device__ __forceinline__ double2 ldv(const GLOBAL_AS double2 *p) {
+    const nat_d2 v = *(const GLOBAL_AS nat_d2 *)p;
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ldv(const GLOBAL_AS float2 *p) {
+    const nat_f2 v = *(const GLOBAL_AS nat_f2 *)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stv(double2 *p, const double2 &v) { *p = v; }
+__device__ __forceinline__ void stv(float2 *p, const float2 &v) { *p = v; }
+__device__ __forceinline__ void stv(GLOBAL_AS double2 *p, const double2 &v) {
+    *(GLOBAL_AS nat_d2 *)p = nat_d2{v.x, v.y};
+}
+__device__ __forceinline__ void stv(GLOBAL_AS float2 *p, const float2 &v) {
+    *(GLOBAL_AS nat_f2 *)p = nat_f2{v.x, v.y};
+}
+
 __host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
 
 template <typename T> size_t plane_lds_bytes(int MB, int KB) {
@@ -75,8 +103,8 @@ template <typename C2> __device__ __forceinline__ C2 czero() {
 }
 
 // psi0[core row m][col] += -i U for the two 16-point blocks of a x/y tile
-template <typename T>
-__device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, typename VT<T>::c2 *psi0, int o, long c0,
+template <typename T, typename P>
+__device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, int o, long c0,
                                                long core1, int ncore, int wave, int kq, int cq,
                                                const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
                                                const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1) {
@@ -85,18 +113,18 @@ __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, typename V
     for (int q = 0; q < 4; ++q) {
         const int m = wave * 16 + VT<T>::row(kq, q);
         if (m >= ncore) continue;
-        C2 *dst = psi0 + (long)o * a.outer_stride + (core1 + m) * a.row_stride + c0;
+        auto dst = psi0 + (long)o * a.outer_stride + (core1 + m) * a.row_stride + c0;
         if (c0 + cq < a.ncols) {
-            C2 v = dst[cq];
+            C2 v = ldv(dst + cq);
             v.x += ai0[q];
             v.y -= ar0[q];
-            dst[cq] = v;
+            stv(dst + cq, v);
         }
         if (c0 + 16 + cq < a.ncols) {
-            C2 v = dst[16 + cq];
+            C2 v = ldv(dst + 16 + cq);
             v.x += ai1[q];
             v.y -= ar1[q];
-            dst[16 + cq] = v;
+            stv(dst + 16 + cq, v);
         }
     }
 }
@@ -132,7 +160,7 @@ __device__ __forceinline__ void line_info(const LineArgs<T> &a, const Clock<T> &
     }
     const int x = (int)qg / a.Ny, y = (int)qg - x * a.Ny;
     li.base = qg * a.Nz;
-    li.gxy = (double)a.g[0][x] * (double)a.g[1][y];
+    li.gxy = (double)glb(a.g[0])[x] * (double)glb(a.g[1])[y];
     li.vrow = -1;
     if (a.use_v && x >= a.vb1[0] && x <= a.vb2[0] && y >= a.vb1[1] && y <= a.vb2[1])
         li.vrow = ((x - a.vb1[0]) * (a.vb2[1] - a.vb1[1] + 1) + (y - a.vb1[1])) * (a.vb2[2] - a.vb1[2] + 1);
@@ -140,9 +168,9 @@ __device__ __forceinline__ void line_info(const LineArgs<T> &a, const Clock<T> &
     const bool inner = x > a.k1b[0] && x < a.k2a[0] && y > a.k1b[1] && y < a.k2a[1];
     li.flags = (inbox ? 1 : 0) | (inner ? 2 : 0);
     if (inbox) {
-        const double px = a.tfP[0][x], py = a.tfP[1][y];
-        const double2 ax = a.tfA[0][x], ay = a.tfA[1][y];
-        const double2 ex = a.tfE[0][x], ey = a.tfE[1][y];
+        const double px = glb(a.tfP[0])[x], py = glb(a.tfP[1])[y];
+        const double2 ax = ldv(glb(a.tfA[0]) + x), ay = ldv(glb(a.tfA[1]) + y);
+        const double2 ex = ldv(glb(a.tfE[0]) + x), ey = ldv(glb(a.tfE[1]) + y);
         li.pxy = px * py;
         li.cxy.x = ax.x * py + ay.x * px;
         li.cxy.y = ax.y * py + ay.y * px;
@@ -163,15 +191,15 @@ struct ZInfo {
 template <typename T>
 __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
     ZInfo zi;
-    zi.gz = (double)a.g[2][z];
+    zi.gz = (double)glb(a.g[2])[z];
     zi.zoff = z - a.vb1[2];
     const bool inv = a.use_v && z >= a.vb1[2] && z <= a.vb2[2];
     const bool inbox = z >= a.k1a[2] && z <= a.k2b[2];
     const bool inner = z > a.k1b[2] && z < a.k2a[2];
     zi.flags = (inv ? 1 : 0) | (inbox ? 2 : 0) | (inner ? 4 : 0);
-    zi.pz = a.tfP[2][z];
-    zi.az = a.tfA[2][z];
-    zi.ez = a.tfE[2][z];
+    zi.pz = glb(a.tfP[2])[z];
+    zi.az = ldv(glb(a.tfA[2]) + z);
+    zi.ez = ldv(glb(a.tfE[2]) + z);
     return zi;
 }
 
@@ -181,16 +209,16 @@ __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
 // phase) and the ABC mask (:1421-1435).
 // One output point of the fused z pass: psi0 at dst, U = alpha D2 (W psi)
 // at the point (alpha folded), pn = psi^n at the point.
-template <typename T>
+template <typename T, typename P>
 __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &clk, const LineInfo &li,
-                                        const ZInfo &zi, typename VT<T>::c2 *dst, T ur, T ui,
+                                        const ZInfo &zi, P dst, T ur, T ui,
                                         typename VT<T>::c2 pn) {
     typedef typename VT<T>::c2 C2;
-    C2 acc = *dst;
+    C2 acc = ldv(dst);
     acc.x += ui;
     acc.y -= ur;
     if ((zi.flags & 1) && li.vrow >= 0) {
-        const T v = clk.vfac * a.vbox[li.vrow + zi.zoff];
+        const T v = clk.vfac * glb(a.vbox)[li.vrow + zi.zoff];
         acc.x += v * pn.y;
         acc.y -= v * pn.x;
     }
@@ -205,7 +233,7 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
     const T gg = (T)(li.gxy * zi.gz);
     acc.x *= gg;
     acc.y *= gg;
-    *dst = acc;
+    stv(dst, acc);
 }
 
 template <typename T>
@@ -215,7 +243,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
                                               const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
                                               const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1) {
     typedef typename VT<T>::c2 C2;
-    C2 *psi0 = reinterpret_cast<C2 *>(a.psi0);
+    auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
 #pragma unroll
     for (int hb = 0; hb < 2; ++hb) {
 #pragma unroll
@@ -243,30 +271,32 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
     const int KB = a.KB, Kp = 4 * KB;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
+    const int ntiles = a.outer_count * a.col_tiles;
+    // blocks past a (smaller) problem's last tile leave before loading anything
+    if (bx >= ntiles) return;
     const int mbr = a.seg.mb[r];
     const bool active = wave < mbr;
-    const T *op = a.op + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
+    auto op = glb(a.op) + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
     T af[KBMAX];
 #pragma unroll
     for (int kb = 0; kb < KBMAX; ++kb) af[kb] = (active && kb < KB) ? op[kb * 64] : T(0);
     const long fft1 = a.seg.fft1[r], core1 = a.seg.core1[r];
     const int ncore = a.seg.ncore[r];
-    const int ntiles = a.outer_count * a.col_tiles;
     const int nel = Kp * PT;
-    const C2 *psi = reinterpret_cast<const C2 *>(a.psi);
-    C2 *psi0 = reinterpret_cast<C2 *>(a.psi0);
+    auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
+    auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
     const int kq = lane >> 4, cq = lane & 15;
     C2 pre[NLD];
     auto fetch = [&](int t) {
         const int o = t / a.col_tiles;
         const long c0 = (long)(t - o * a.col_tiles) * PT;
-        const C2 *src = psi + (long)o * a.outer_stride + fft1 * a.row_stride + c0;
+        auto src = psi + (long)o * a.outer_stride + fft1 * a.row_stride + c0;
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
             const int i = tid + j * nthr;
             const int l = i / PT, c = i % PT;
             C2 v = czero<C2>();
-            if (i < nel && l < a.L && c0 + c < a.ncols) v = src[(long)l * a.row_stride + c];
+            if (i < nel && l < a.L && c0 + c < a.ncols) v = ldv(src + (long)l * a.row_stride + c);
             pre[j] = v;
         }
     };
@@ -323,6 +353,7 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
     LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)LT * LS * sizeof(C2)));
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
+    if (bx >= a.line_tiles) return;
     const int mbr = a.seg.mb[r];
     const long fft1 = a.seg.fft1[r];
     const int core1 = a.seg.core1[r], ncore = a.seg.ncore[r];
@@ -335,12 +366,12 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
     const bool lane_ok = active && m < ncore;
     const int z = core1 + (lane_ok ? m : 0);
     const ZInfo zi = z_info<T>(a, z);
-    const T *op = a.op + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
+    auto op = glb(a.op) + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
     T af[KBMAX];
 #pragma unroll
     for (int kb = 0; kb < KBMAX; ++kb) af[kb] = (active && kb < KB) ? op[kb * 64] : T(0);
     const int nel = LT * Kp;
-    const C2 *psi = reinterpret_cast<const C2 *>(a.psi);
+    auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
     C2 pre[NLD];
     auto fetch = [&](int t) {
         const long q0 = (long)t * LT;
@@ -349,7 +380,7 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
             const int i = tid + j * nthr;
             const int line = i / Kp, l = i - line * Kp;
             C2 v = czero<C2>();
-            if (i < nel && l < a.L && q0 + line < a.nlines) v = psi[(q0 + line) * a.Nz + fft1 + l];
+            if (i < nel && l < a.L && q0 + line < a.nlines) v = ldv(psi + (q0 + line) * a.Nz + fft1 + l);
             pre[j] = v;
         }
     };
@@ -401,19 +432,74 @@ __global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
 // per-step time scalars of the z pass come from an FP64 table indexed by a
 // device step counter, so a captured graph replays any number of steps.
 // ---------------------------------------------------------------------------
+// Each block copies the fields of its problem that the pass reads, with
+// segment r moved to slot 0, into a local argument block: the body then works
+// on values loaded once at entry (as from kernel arguments) instead of
+// re-reading the table inside its tile loop.
 template <typename T>
-__global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *list, const int2 *map) {
-    const int2 br = map[blockIdx.y];
-    plane_reg_body<T>(list[br.x], br.y, blockIdx.x, gridDim.x);
+__device__ __forceinline__ void seg_slice(SegTable &d, const SegTable &s, int r) {
+    d.fft1[0] = s.fft1[r];
+    d.core1[0] = s.core1[r];
+    d.ncore[0] = s.ncore[r];
+    d.mb[0] = s.mb[r];
+    d.opoff[0] = s.opoff[r];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *list, const int2 *map, const double *clock,
-                                                    const int *step, int nprob) {
+__global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restrict__ list,
+                                                     const int2 *__restrict__ map) {
+    const int2 br = map[blockIdx.y];
+    const PlaneArgs<T> &g = list[br.x];
+    PlaneArgs<T> a;
+    a.psi = g.psi;
+    a.psi0 = g.psi0;
+    a.op = g.op;
+    a.row_stride = g.row_stride;
+    a.outer_stride = g.outer_stride;
+    a.outer_count = g.outer_count;
+    a.ncols = g.ncols;
+    a.col_tiles = g.col_tiles;
+    a.L = g.L;
+    a.KB = g.KB;
+    seg_slice<T>(a.seg, g.seg, br.y);
+    plane_reg_body<T>(a, 0, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
+                                                    const double *__restrict__ clock, const int *__restrict__ step,
+                                                    int nprob) {
     const int2 br = map[blockIdx.y];
     const double *c = clock + ((long)(*step) * nprob + br.x) * 3;
     const Clock<T> clk = {c[1], c[2], (T)c[0]};
-    line_reg_body<T>(list[br.x], clk, br.y, blockIdx.x, gridDim.x);
+    const LineArgs<T> &g = list[br.x];
+    LineArgs<T> a;
+    a.psi = g.psi;
+    a.psi0 = g.psi0;
+    a.op = g.op;
+    a.nlines = g.nlines;
+    a.Ny = g.Ny;
+    a.Nz = g.Nz;
+    a.L = g.L;
+    a.KB = g.KB;
+    a.line_tiles = g.line_tiles;
+    seg_slice<T>(a.seg, g.seg, br.y);
+    a.vbox = g.vbox;
+    a.use_v = g.use_v;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.vb1[d] = g.vb1[d];
+        a.vb2[d] = g.vb2[d];
+        a.tfP[d] = g.tfP[d];
+        a.tfA[d] = g.tfA[d];
+        a.tfE[d] = g.tfE[d];
+        a.k1a[d] = g.k1a[d];
+        a.k1b[d] = g.k1b[d];
+        a.k2a[d] = g.k2a[d];
+        a.k2b[d] = g.k2b[d];
+        a.g[d] = g.g[d];
+    }
+    line_reg_body<T>(a, clk, 0, blockIdx.x, gridDim.x);
 }
 
 __global__ void k_tick(int *step) {

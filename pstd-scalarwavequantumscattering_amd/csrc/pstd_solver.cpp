@@ -834,6 +834,7 @@ struct pstd_batch {
     int2 *maps[3] = {nullptr, nullptr, nullptr};
     int nmap[3] = {0, 0, 0};
     int gx[3] = {8, 8, 8};
+    int tiles_per_block = 16;
     double *clock = nullptr;      // [step][problem][vfac, sin t, cos t]
     long clock_cap = 0;
     int *step = nullptr;          // steps done since creation (device)
@@ -917,10 +918,11 @@ int batch_build(pstd_batch *b) {
         b->nmap[k] = (int)maps[k].size();
         HIPCHK(hipMalloc(&b->maps[k], maps[k].size() * sizeof(int2)), "batch map");
         HIPCHK(hipMemcpy(b->maps[k], maps[k].data(), maps[k].size() * sizeof(int2), hipMemcpyHostToDevice), "batch map");
-        // about four tiles per block for the largest problem; smaller problems
-        // leave their surplus blocks idle, and the many blocks balance
-        // dynamically over the CUs
-        long g = (max_tiles[k] + 3) / 4;
+        // about sixteen tiles per block for the largest problem (so the
+        // prefetch pipeline of a block reaches its steady state); smaller
+        // problems leave their surplus blocks idle, and the blocks of all
+        // problems balance dynamically over the CUs
+        long g = (max_tiles[k] + b->tiles_per_block - 1) / b->tiles_per_block;
         g = std::max(8L, (g + 7) / 8 * 8);
         b->gx[k] = (int)g;
     }
