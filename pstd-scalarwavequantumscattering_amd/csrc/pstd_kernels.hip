@@ -15,6 +15,8 @@
 //          (L <= 96, core <= 128); the psi tile is double-buffered through
 //          registers so the next tile's loads overlap this tile's MFMAs.
 //   *_lds  operator fragments in LDS; any L whose tiles fit on chip.
+#include <vector>
+
 #include "pstd_kernels.hpp"
 
 namespace pstd {
@@ -48,7 +50,14 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
 constexpr int PT = 32;      // complex points per x/y tile (two 16-point blocks)
 constexpr int LT = 32;      // lines per z tile (two 16-line blocks)
 constexpr int KBMAX = 24;   // register variant: L <= 96
-constexpr int NLD = (4 * KBMAX * PT + 255) / 256;   // tile loads per thread (>= 256 threads)
+// The register variants are instantiated for a compile-time number of 4-row
+// k blocks KBC (16, 20, 24: L = 64, 80, 96), which sizes the operator
+// fragments and the tile prefetch exactly, and for KBC = 0 (any KB <= KBMAX,
+// read at run time).
+template <int KBC> struct KbShape {
+    static constexpr int KBM = KBC ? KBC : KBMAX;               // fragments per lane
+    static constexpr int NL = (4 * KBM * PT + 255) / 256;       // tile loads per thread (>= 256 threads)
+};
 
 // Pointers read from a device-side argument table (the batched passes) are
 // generic to the compiler, which then emits flat memory instructions; those
@@ -59,6 +68,7 @@ constexpr int NLD = (4 * KBMAX * PT + 255) / 256;   // tile loads per thread (>=
 template <typename P> __device__ __forceinline__ GLOBAL_AS P *glb(P *p) { return (GLOBAL_AS P *)p; }
 typedef double nat_d2 __attribute__((ext_vector_type(2)));
 typedef float nat_f2 __attribute__((ext_vector_type(2)));
+typedef int nat_i4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double2 ldv(const double2 *p) { return *p; }
 __device__ __forceinline__ float2 ldv(const float2 *p) { return *p; }
 __device__ __forceinline__ double2 ldv(const GLOBAL_AS double2 *p) {
@@ -84,14 +94,16 @@ template <typename T> size_t plane_lds_bytes(int MB, int KB) {
     return align16((size_t)MB * KB * 64 * sizeof(T)) + (size_t)4 * KB * PT * 2 * sizeof(T);
 }
 struct LineInfo;
-constexpr size_t LINEINFO_BYTES = 64;
+constexpr size_t LINEINFO_BYTES = LINE_RECORD_BYTES;
 template <typename T> size_t line_lds_bytes(int MB, int KB) {
     return align16((size_t)MB * KB * 64 * sizeof(T)) + align16((size_t)LT * (4 * KB + 1) * 2 * sizeof(T)) +
            LT * LINEINFO_BYTES;
 }
 template <typename T> size_t plane_reg_lds_bytes(int KB) { return (size_t)4 * KB * PT * 2 * sizeof(T); }
+constexpr size_t ZSRC_BYTES = 48;
+constexpr int ZSRC_ROWS = 128;   // 16-row blocks (<= 8) x 16 core rows
 template <typename T> size_t line_reg_lds_bytes(int KB) {
-    return align16((size_t)LT * (4 * KB + 1) * 2 * sizeof(T)) + LT * LINEINFO_BYTES;
+    return align16((size_t)LT * (4 * KB + 1) * 2 * sizeof(T)) + LT * LINEINFO_BYTES + ZSRC_ROWS * ZSRC_BYTES;
 }
 bool reg_variant_ok(int MB, int KB) { return KB <= KBMAX && MB <= 8; }
 
@@ -129,13 +141,14 @@ __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, in
     }
 }
 
-// Per-line terms of the z-pass epilogue, built once per tile in LDS.  With
-// the separable TF/SF tables the source is
+// Per-line terms of the z-pass epilogue, held in LDS per tile.  With the
+// separable TF/SF tables the source is
 //   S = E_x E_y E_z ((A_x P_y + P_x A_y) P_z + P_x P_y A_z),
 // and the time factor (sin t + i cos t) of PSTD_QScat.cpp:1238-1239 is folded
-// into exy.  All FP64.
+// into exy (LDS and FFT variants, built per tile) or into E_z (register
+// variant, whose records come from a time-independent table).  All FP64.
 struct LineInfo {
-    double2 exy;    // E_x E_y (sin t + i cos t)
+    double2 exy;    // E_x E_y, times (sin t + i cos t) where folded here
     double2 cxy;    // A_x P_y + P_x A_y
     double pxy;     // P_x P_y
     double gxy;     // g1(x) g2(y)
@@ -180,13 +193,19 @@ __device__ __forceinline__ void line_info(const LineArgs<T> &a, const Clock<T> &
     }
 }
 
-// Per-lane z terms (z is fixed per lane for a block's whole life).
+// Per-lane z terms (z is fixed per lane for a block's whole life): the part
+// every point needs stays in registers, the TF/SF part (read only in the
+// source shell) may live in LDS.
 struct ZInfo {
-    double2 az, ez;
-    double pz, gz;
+    double gz;
     int zoff;       // z - vb1z
     int flags;      // bit 0: in potential box, bit 1: in TF box, bit 2: strictly inside
 };
+struct ZSrc {
+    double2 az, ez;
+    double pz, pad;
+};
+static_assert(sizeof(ZSrc) == ZSRC_BYTES, "ZSrc must fit its LDS slot");
 
 template <typename T>
 __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
@@ -197,10 +216,17 @@ __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
     const bool inbox = z >= a.k1a[2] && z <= a.k2b[2];
     const bool inner = z > a.k1b[2] && z < a.k2a[2];
     zi.flags = (inv ? 1 : 0) | (inbox ? 2 : 0) | (inner ? 4 : 0);
-    zi.pz = glb(a.tfP[2])[z];
-    zi.az = ldv(glb(a.tfA[2]) + z);
-    zi.ez = ldv(glb(a.tfE[2]) + z);
     return zi;
+}
+
+template <typename T>
+__device__ __forceinline__ ZSrc z_src(const LineArgs<T> &a, int z) {
+    ZSrc zs;
+    zs.pz = glb(a.tfP[2])[z];
+    zs.az = ldv(glb(a.tfA[2]) + z);
+    zs.ez = ldv(glb(a.tfE[2]) + z);
+    zs.pad = 0;
+    return zs;
 }
 
 // z-pass epilogue: psi0 = g (psi0 - i U - i f V~ psi - S i e^{-it}), the
@@ -211,7 +237,7 @@ __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
 // at the point (alpha folded), pn = psi^n at the point.
 template <typename T, typename P>
 __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &clk, const LineInfo &li,
-                                        const ZInfo &zi, P dst, T ur, T ui,
+                                        const ZInfo &zi, const ZSrc &zs, P dst, T ur, T ui,
                                         typename VT<T>::c2 pn) {
     typedef typename VT<T>::c2 C2;
     C2 acc = ldv(dst);
@@ -223,10 +249,12 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
         acc.y -= v * pn.x;
     }
     if ((zi.flags & 2) && (li.flags & 1) && !((zi.flags & 4) && (li.flags & 2))) {
-        const double ur2 = li.cxy.x * zi.pz + li.pxy * zi.az.x;
-        const double ui2 = li.cxy.y * zi.pz + li.pxy * zi.az.y;
-        const double er = li.exy.x * zi.ez.x - li.exy.y * zi.ez.y;
-        const double ei = li.exy.x * zi.ez.y + li.exy.y * zi.ez.x;
+        const double2 az = zs.az, ez = zs.ez;
+        const double pz = zs.pz;
+        const double ur2 = li.cxy.x * pz + li.pxy * az.x;
+        const double ui2 = li.cxy.y * pz + li.pxy * az.y;
+        const double er = li.exy.x * ez.x - li.exy.y * ez.y;
+        const double ei = li.exy.x * ez.y + li.exy.y * ez.x;
         acc.x -= (T)(er * ur2 - ei * ui2);
         acc.y -= (T)(er * ui2 + ei * ur2);
     }
@@ -239,7 +267,7 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
 template <typename T>
 __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<T> &clk, const typename VT<T>::c2 *Ls,
                                               const LineInfo *LI,
-                                              int LS, const ZInfo &zi, int z, int m, int i1, int kq,
+                                              int LS, const ZInfo &zi, const ZSrc &zs, int z, int m, int i1, int kq,
                                               const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
                                               const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1) {
     typedef typename VT<T>::c2 C2;
@@ -251,7 +279,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
             const int line = hb * 16 + VT<T>::row(kq, q);
             const LineInfo &li = LI[line];
             if (li.base < 0) continue;
-            z_point<T>(a, clk, li, zi, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
+            z_point<T>(a, clk, li, zi, zs, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
                        Ls[line * LS + i1 + m]);
         }
     }
@@ -262,13 +290,14 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
 // strided set of column tiles; wave w owns 16-row block w of the core and
 // keeps its operator fragments in registers for the life of the block.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int KBC>
 __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
+    constexpr int KBM = KbShape<KBC>::KBM, NLD = KbShape<KBC>::NL;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     C2 *Bs = reinterpret_cast<C2 *>(smem);
-    const int KB = a.KB, Kp = 4 * KB;
+    const int KB = KBC ? KBC : a.KB, Kp = 4 * KB;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int ntiles = a.outer_count * a.col_tiles;
@@ -277,9 +306,9 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
     const int mbr = a.seg.mb[r];
     const bool active = wave < mbr;
     auto op = glb(a.op) + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
-    T af[KBMAX];
+    T af[KBM];
 #pragma unroll
-    for (int kb = 0; kb < KBMAX; ++kb) af[kb] = (active && kb < KB) ? op[kb * 64] : T(0);
+    for (int kb = 0; kb < KBM; ++kb) af[kb] = (active && (KBC || kb < KB)) ? op[kb * 64] : T(0);
     const long fft1 = a.seg.fft1[r], core1 = a.seg.core1[r];
     const int ncore = a.seg.ncore[r];
     const int nel = Kp * PT;
@@ -317,8 +346,8 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
             ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
             const C2 *Bw = Bs + kq * PT + cq;
 #pragma unroll
-            for (int kb = 0; kb < KBMAX; ++kb) {
-                if (kb < KB) {
+            for (int kb = 0; kb < KBM; ++kb) {
+                if (KBC || kb < KB) {
                     const C2 b0 = Bw[kb * 4 * PT];
                     const C2 b1 = Bw[kb * 4 * PT + 16];
                     ar0 = mfma16(af[kb], b0.x, ar0);
@@ -332,9 +361,16 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
-    plane_reg_body<T>(a, blockIdx.y, blockIdx.x, gridDim.x);
+// Waves per SIMD the register variants are compiled for: the FP32 z pass
+// fits three (168 VGPRs) when its fragments are sized at compile time.
+template <typename T, int KBC> constexpr int line_waves() { return (sizeof(T) == 4 && KBC && KBC <= 20) ? 3 : 1; }
+// The FP32 x/y pass at KB = 20 is compiled for four (128 VGPRs, a few
+// spills, measured slightly faster than three); FP64 spills too much to gain.
+template <typename T, int KBC> constexpr int plane_waves() { return (sizeof(T) == 4 && KBC == 20) ? 4 : 1; }
+
+template <typename T, int KBC>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(plane_waves<T, KBC>()))) k_lap_plane_reg(PlaneArgs<T> a) {
+    plane_reg_body<T, KBC>(a, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -343,14 +379,16 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
 // operand, the operator the B operand) so that each lane ends with both
 // components of one output point and 16 lanes cover 16 consecutive z.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int KBC>
 __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<T> &clk, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
+    constexpr int KBM = KbShape<KBC>::KBM, NLD = KbShape<KBC>::NL;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int KB = a.KB, Kp = 4 * KB, LS = Kp + 1;
+    const int KB = KBC ? KBC : a.KB, Kp = 4 * KB, LS = Kp + 1;
     C2 *Ls = reinterpret_cast<C2 *>(smem);
     LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)LT * LS * sizeof(C2)));
+    ZSrc *ZT = reinterpret_cast<ZSrc *>(smem + align16((size_t)LT * LS * sizeof(C2)) + LT * LINEINFO_BYTES);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
     if (bx >= a.line_tiles) return;
@@ -366,15 +404,31 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
     const bool lane_ok = active && m < ncore;
     const int z = core1 + (lane_ok ? m : 0);
     const ZInfo zi = z_info<T>(a, z);
+    // TF/SF z terms of the block's core rows with the step's time factor
+    // (sin t + i cos t) folded into E_z; visible after the first barrier
+    if (tid < ncore && tid < ZSRC_ROWS) {
+        ZSrc zs = z_src<T>(a, core1 + tid);
+        const double2 e = zs.ez;
+        zs.ez.x = e.x * clk.sin_t - e.y * clk.cos_t;
+        zs.ez.y = e.x * clk.cos_t + e.y * clk.sin_t;
+        ZT[tid] = zs;
+    }
     auto op = glb(a.op) + a.seg.opoff[r] + (long)(active ? wave : 0) * KB * 64 + lane;
-    T af[KBMAX];
+    T af[KBM];
 #pragma unroll
-    for (int kb = 0; kb < KBMAX; ++kb) af[kb] = (active && kb < KB) ? op[kb * 64] : T(0);
+    for (int kb = 0; kb < KBM; ++kb) af[kb] = (active && (KBC || kb < KB)) ? op[kb * 64] : T(0);
     const int nel = LT * Kp;
     auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
+    auto rec = glb(reinterpret_cast<const nat_i4 *>(a.linfo));
     C2 pre[NLD];
+    nat_i4 lpre = {0, 0, 0, 0};   // 16-byte quarter tid % 4 of line record tid / 4
     auto fetch = [&](int t) {
         const long q0 = (long)t * LT;
+        if (tid < 4 * LT) {
+            const long q = q0 + (tid >> 2);
+            if (q < a.nlines) lpre = rec[q * 4 + (tid & 3)];
+            else lpre = ((tid & 3) == 3) ? nat_i4{-1, -1, -1, 0} : nat_i4{0, 0, 0, 0};   // base = -1
+        }
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
             const int i = tid + j * nthr;
@@ -387,7 +441,6 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
     int t = bx;
     if (t < a.line_tiles) fetch(t);
     for (; t < a.line_tiles; t += gx) {
-        const long q0 = (long)t * LT;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -397,7 +450,7 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
                 Ls[line * LS + l] = pre[j];
             }
         }
-        if (tid < LT) line_info<T>(a, clk, q0 + tid, LI[tid]);
+        if (tid < 4 * LT) reinterpret_cast<nat_i4 *>(LI)[tid] = lpre;
         __syncthreads();
         if (t + gx < a.line_tiles) fetch(t + gx);
         if (!active) continue;
@@ -405,8 +458,8 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
         const C2 *L0 = Ls + cq * LS + kq;
         const C2 *L1 = Ls + (16 + cq) * LS + kq;
 #pragma unroll
-        for (int kb = 0; kb < KBMAX; ++kb) {
-            if (kb < KB) {
+        for (int kb = 0; kb < KBM; ++kb) {
+            if (KBC || kb < KB) {
                 const C2 p0 = L0[4 * kb];
                 const C2 p1 = L1[4 * kb];
                 ar0 = mfma16(p0.x, af[kb], ar0);
@@ -415,14 +468,15 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
                 ai1 = mfma16(p1.y, af[kb], ai1);
             }
         }
-        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
+template <typename T, int KBC>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(line_waves<T, KBC>())))
+k_lap_line_reg(LineArgs<T> a) {
     const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
-    line_reg_body<T>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
+    line_reg_body<T, KBC>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,7 +499,7 @@ __device__ __forceinline__ void seg_slice(SegTable &d, const SegTable &s, int r)
     d.opoff[0] = s.opoff[r];
 }
 
-template <typename T>
+template <typename T, int KBC>
 __global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restrict__ list,
                                                      const int2 *__restrict__ map) {
     const int2 br = map[blockIdx.y];
@@ -462,11 +516,12 @@ __global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restr
     a.L = g.L;
     a.KB = g.KB;
     seg_slice<T>(a.seg, g.seg, br.y);
-    plane_reg_body<T>(a, 0, blockIdx.x, gridDim.x);
+    plane_reg_body<T, KBC>(a, 0, blockIdx.x, gridDim.x);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
+template <typename T, int KBC>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(line_waves<T, KBC>())))
+k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
                                                     const double *__restrict__ clock, const int *__restrict__ step,
                                                     int nprob) {
     const int2 br = map[blockIdx.y];
@@ -483,6 +538,7 @@ __global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restric
     a.L = g.L;
     a.KB = g.KB;
     a.line_tiles = g.line_tiles;
+    a.linfo = g.linfo;
     seg_slice<T>(a.seg, g.seg, br.y);
     a.vbox = g.vbox;
     a.use_v = g.use_v;
@@ -499,7 +555,22 @@ __global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restric
         a.k2b[d] = g.k2b[d];
         a.g[d] = g.g[d];
     }
-    line_reg_body<T>(a, clk, 0, blockIdx.x, gridDim.x);
+    line_reg_body<T, KBC>(a, clk, 0, blockIdx.x, gridDim.x);
+}
+
+// Time-independent per-line terms (line_info at unit time factor, so that
+// exy = E_x E_y); the register z pass reads them per tile.
+template <typename T>
+__global__ void __launch_bounds__(256) k_line_table(LineArgs<T> a, LineInfo *tab) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nlines) return;
+    const Clock<T> unit = {1.0, 0.0, T(0)};
+    LineInfo li;
+    li.exy = make_double2(0, 0);
+    li.cxy = make_double2(0, 0);
+    li.pxy = 0;
+    line_info<T>(a, unit, q, li);
+    tab[q] = li;
 }
 
 __global__ void k_tick(int *step) {
@@ -585,6 +656,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
     const bool lane_ok = wave < mbr && m < ncore;
     const int z = core1 + (lane_ok ? m : 0);
     const ZInfo zi = z_info<T>(a, z);
+    const ZSrc zs = z_src<T>(a, z);
     for (int t = blockIdx.x; t < a.line_tiles; t += gridDim.x) {
         const long q0 = (long)t * LT;
         __syncthreads();
@@ -610,7 +682,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             ar1 = mfma16(p1.x, af, ar1);
             ai1 = mfma16(p1.y, af, ai1);
         }
-        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
 }
 
@@ -898,7 +970,7 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
             const int z = core1 + m;
             const ZInfo zi = z_info<T>(a, z);
             const C2 u = S[(i1 + m) * LDW + c];
-            z_point<T>(a, clk, li, zi, psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
+            z_point<T>(a, clk, li, zi, z_src<T>(a, z), psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
         }
     }
 }
@@ -915,6 +987,40 @@ template <typename T> int fft_occupancy(int L, bool line) {
 }
 
 // ---------------------------------------------------------------------------
+// Register-variant instantiation for a k-block count.
+template <typename T> const void *plane_reg_kernel(int KB) {
+    switch (KB) {
+    case 16: return (const void *)k_lap_plane_reg<T, 16>;
+    case 20: return (const void *)k_lap_plane_reg<T, 20>;
+    case 24: return (const void *)k_lap_plane_reg<T, 24>;
+    default: return (const void *)k_lap_plane_reg<T, 0>;
+    }
+}
+template <typename T> const void *line_reg_kernel(int KB) {
+    switch (KB) {
+    case 16: return (const void *)k_lap_line_reg<T, 16>;
+    case 20: return (const void *)k_lap_line_reg<T, 20>;
+    case 24: return (const void *)k_lap_line_reg<T, 24>;
+    default: return (const void *)k_lap_line_reg<T, 0>;
+    }
+}
+template <typename T> const void *batch_plane_kernel(int KB) {
+    switch (KB) {
+    case 16: return (const void *)k_batch_plane<T, 16>;
+    case 20: return (const void *)k_batch_plane<T, 20>;
+    case 24: return (const void *)k_batch_plane<T, 24>;
+    default: return (const void *)k_batch_plane<T, 0>;
+    }
+}
+template <typename T> const void *batch_line_kernel(int KB) {
+    switch (KB) {
+    case 16: return (const void *)k_batch_line<T, 16>;
+    case 20: return (const void *)k_batch_line<T, 20>;
+    case 24: return (const void *)k_batch_line<T, 24>;
+    default: return (const void *)k_batch_line<T, 0>;
+    }
+}
+
 template <typename T>
 hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
     dim3 grid(grid_x, a.seg.n);
@@ -922,7 +1028,9 @@ hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
         hipLaunchKernelGGL(k_fft_plane<T>, grid, dim3(FNTHR), fft_lds_bytes<T>(a.L), s, a);
     } else if (reg_variant_ok(a.MB, a.KB)) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
-        hipLaunchKernelGGL(k_lap_plane_reg<T>, grid, block, plane_reg_lds_bytes<T>(a.KB), s, a);
+        PlaneArgs<T> arg = a;
+        void *args[] = {&arg};
+        return hipLaunchKernel(plane_reg_kernel<T>(a.KB), grid, block, args, plane_reg_lds_bytes<T>(a.KB), s);
     } else {
         dim3 block(64 * a.MB);
         hipLaunchKernelGGL(k_lap_plane_lds<T>, grid, block, plane_lds_bytes<T>(a.MB, a.KB), s, a);
@@ -936,7 +1044,9 @@ hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s) {
         hipLaunchKernelGGL(k_fft_line<T>, grid, dim3(FNTHR), fft_lds_bytes<T>(a.L), s, a);
     } else if (reg_variant_ok(a.MB, a.KB)) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
-        hipLaunchKernelGGL(k_lap_line_reg<T>, grid, block, line_reg_lds_bytes<T>(a.KB), s, a);
+        LineArgs<T> arg = a;
+        void *args[] = {&arg};
+        return hipLaunchKernel(line_reg_kernel<T>(a.KB), grid, block, args, line_reg_lds_bytes<T>(a.KB), s);
     } else {
         dim3 block(64 * a.MB);
         hipLaunchKernelGGL(k_lap_line_lds<T>, grid, block, line_lds_bytes<T>(a.MB, a.KB), s, a);
@@ -952,22 +1062,29 @@ hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStre
 
 template <typename T>
 hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_plane<T>, dim3(grid_x, nmap), dim3(block), plane_reg_lds_bytes<T>(KB), s, list, map);
-    return hipGetLastError();
+                              bool kb_uniform, hipStream_t s) {
+    void *args[] = {&list, &map};
+    return hipLaunchKernel(batch_plane_kernel<T>(kb_uniform ? KB : 0), dim3(grid_x, nmap), dim3(block), args,
+                           plane_reg_lds_bytes<T>(KB), s);
 }
 template <typename T>
 hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
-                             int nprob, int grid_x, int block, int KB, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_line<T>, dim3(grid_x, nmap), dim3(block), line_reg_lds_bytes<T>(KB), s, list, map, clock,
-                       step, nprob);
-    return hipGetLastError();
+                             int nprob, int grid_x, int block, int KB, bool kb_uniform, hipStream_t s) {
+    void *args[] = {&list, &map, &clock, &step, &nprob};
+    return hipLaunchKernel(batch_line_kernel<T>(kb_uniform ? KB : 0), dim3(grid_x, nmap), dim3(block), args,
+                           line_reg_lds_bytes<T>(KB), s);
 }
 hipError_t launch_tick(int *step, hipStream_t s) {
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, step);
     return hipGetLastError();
 }
 bool batch_variant_ok(int MB, int KB) { return reg_variant_ok(MB, KB); }
+
+template <typename T> hipError_t launch_line_table(const LineArgs<T> &a, void *table, hipStream_t s) {
+    const int blocks = (int)((a.nlines + 255) / 256);
+    hipLaunchKernelGGL(k_line_table<T>, dim3(blocks), dim3(256), 0, s, a, reinterpret_cast<LineInfo *>(table));
+    return hipGetLastError();
+}
 
 // Resident blocks per CU of the GEMM pass kernel that launch_plane /
 // launch_line would pick for (MB, KB), from the runtime's occupancy calculator.
@@ -977,9 +1094,10 @@ template <typename T> int pass_occupancy(int MB, int KB, bool line) {
     if (reg_variant_ok(MB, KB)) {
         const int block = 64 * (MB > 4 ? MB : 4);
         if (line)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lap_line_reg<T>, block, line_reg_lds_bytes<T>(KB));
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, line_reg_kernel<T>(KB), block, line_reg_lds_bytes<T>(KB));
         else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lap_plane_reg<T>, block, plane_reg_lds_bytes<T>(KB));
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, plane_reg_kernel<T>(KB), block,
+                                                             plane_reg_lds_bytes<T>(KB));
     } else {
         const int block = 64 * MB;
         if (line)
@@ -1002,14 +1120,20 @@ bool gemm_variant_fits(int MB, int KB, size_t esz, size_t max_lds) {
 }
 
 hipError_t configure_kernels(int max_lds) {
-    const void *k[] = {(const void *)k_lap_plane_reg<double>, (const void *)k_lap_plane_reg<float>,
-                       (const void *)k_lap_line_reg<double>,  (const void *)k_lap_line_reg<float>,
-                       (const void *)k_lap_plane_lds<double>, (const void *)k_lap_plane_lds<float>,
-                       (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>,
-                       (const void *)k_fft_plane<double>,     (const void *)k_fft_plane<float>,
-                       (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>,
-                       (const void *)k_batch_plane<double>,   (const void *)k_batch_plane<float>,
-                       (const void *)k_batch_line<double>,    (const void *)k_batch_line<float>};
+    std::vector<const void *> k = {(const void *)k_lap_plane_lds<double>, (const void *)k_lap_plane_lds<float>,
+                                   (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>,
+                                   (const void *)k_fft_plane<double>,     (const void *)k_fft_plane<float>,
+                                   (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>};
+    for (int kb : {0, 16, 20, 24}) {
+        k.push_back(plane_reg_kernel<double>(kb));
+        k.push_back(plane_reg_kernel<float>(kb));
+        k.push_back(line_reg_kernel<double>(kb));
+        k.push_back(line_reg_kernel<float>(kb));
+        k.push_back(batch_plane_kernel<double>(kb));
+        k.push_back(batch_plane_kernel<float>(kb));
+        k.push_back(batch_line_kernel<double>(kb));
+        k.push_back(batch_line_kernel<float>(kb));
+    }
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return e;
@@ -1018,6 +1142,8 @@ hipError_t configure_kernels(int max_lds) {
 }
 
 template hipError_t launch_plane<double>(const PlaneArgs<double> &, int, hipStream_t);
+template hipError_t launch_line_table<double>(const LineArgs<double> &, void *, hipStream_t);
+template hipError_t launch_line_table<float>(const LineArgs<float> &, void *, hipStream_t);
 template hipError_t launch_plane<float>(const PlaneArgs<float> &, int, hipStream_t);
 template hipError_t launch_line<double>(const LineArgs<double> &, int, hipStream_t);
 template hipError_t launch_line<float>(const LineArgs<float> &, int, hipStream_t);
@@ -1031,11 +1157,13 @@ template size_t fft_lds_bytes<double>(int);
 template size_t fft_lds_bytes<float>(int);
 template int fft_occupancy<double>(int, bool);
 template int fft_occupancy<float>(int, bool);
-template hipError_t launch_batch_plane<double>(const PlaneArgs<double> *, const int2 *, int, int, int, int, hipStream_t);
-template hipError_t launch_batch_plane<float>(const PlaneArgs<float> *, const int2 *, int, int, int, int, hipStream_t);
+template hipError_t launch_batch_plane<double>(const PlaneArgs<double> *, const int2 *, int, int, int, int, bool,
+                                               hipStream_t);
+template hipError_t launch_batch_plane<float>(const PlaneArgs<float> *, const int2 *, int, int, int, int, bool,
+                                              hipStream_t);
 template hipError_t launch_batch_line<double>(const LineArgs<double> *, const int2 *, int, const double *, const int *,
-                                              int, int, int, int, hipStream_t);
+                                              int, int, int, int, bool, hipStream_t);
 template hipError_t launch_batch_line<float>(const LineArgs<float> *, const int2 *, int, const double *, const int *,
-                                             int, int, int, int, hipStream_t);
+                                             int, int, int, int, bool, hipStream_t);
 
 }  // namespace pstd

@@ -77,6 +77,9 @@ struct LineArgs {
     double sin_t, cos_t;
     // ABC
     const T *g[3];
+    // time-independent per-line terms (64-byte records, nlines of them) built
+    // once by launch_line_table; read by the register variant
+    const void *linfo;
     int method;          // PSTD_METHOD_GEMM or PSTD_METHOD_FFT
     int fline_tiles;     // line tiles of the FFT variant
     FftPlan<T> fft;
@@ -105,13 +108,17 @@ template <typename T> size_t fft_lds_bytes(int L);
 template <typename T> int fft_occupancy(int L, bool line);
 bool gemm_variant_fits(int MB, int KB, size_t esz, size_t max_lds);
 bool batch_variant_ok(int MB, int KB);
+// KB sizes the LDS tile (the largest of the batch); kb_uniform says every
+// problem has that KB, so the compile-time instantiation may be used.
 template <typename T>
 hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
-                              hipStream_t s);
+                              bool kb_uniform, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
-                             int nprob, int grid_x, int block, int KB, hipStream_t s);
+                             int nprob, int grid_x, int block, int KB, bool kb_uniform, hipStream_t s);
 hipError_t launch_tick(int *step, hipStream_t s);
+constexpr size_t LINE_RECORD_BYTES = 64;
+template <typename T> hipError_t launch_line_table(const LineArgs<T> &a, void *table, hipStream_t s);
 constexpr int FFT_NC = 16;       // lines per FFT tile
 constexpr int FFT_LMAX = 512;    // longest local line of the FFT variant
 hipError_t configure_kernels(int max_lds);

@@ -67,6 +67,7 @@ struct pstd_solver {
     void *vbox = nullptr;
     long nvbox = 0;
     bool has_v = false;
+    void *linfo = nullptr;        // per-line z-pass records (register variant)
     double *surf_w = nullptr;     // 6 faces x (ps, fd) x L_a
     double2 *surf_acc = nullptr;  // 6 faces x 3 x plane
     SurfaceOp sop[6];
@@ -137,6 +138,8 @@ int select_device(pstd_solver *s) {
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     return 0;
 }
+
+int build_line_table(pstd_solver *s);
 
 int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd_potential_fn hook,
                   pstd_solver **out) {
@@ -241,6 +244,7 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
     if ((rc = s->upload(s->surf_w, w.data(), w.size() * 8, "surface weights"))) { delete s; return rc; }
     if ((rc = s->alloc(s->surf_acc, acc_n * 16, "surface accumulators"))) { delete s; return rc; }
     HIPCHK(hipMemset(s->surf_acc, 0, acc_n * 16), "hipMemset");
+    if ((rc = build_line_table(s))) { delete s; return rc; }
     s->nTau = 1;   // PSTD_QScat.cpp:1117
     s->nSur = 0;
     HIPCHK(hipDeviceSynchronize(), "init");
@@ -351,9 +355,26 @@ LineArgs<T> line_args(pstd_solver *s, const T *psi, T *psi0, long ntau) {
         a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)std::max(g.ksi1b[d], g.ksi2b[d]);
         a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
     }
+    a.linfo = s->linfo;
     step_clock(s, ntau, a.vfac, a.sin_t, a.cos_t);
     a.use_v = (s->has_v && s->nvbox > 0) ? 1 : 0;
     return a;
+}
+
+// Per-line terms of the z pass (ABC product, potential-box row, TF/SF x-y
+// factors), time-independent: built once on the device.
+template <typename T>
+int build_line_table_t(pstd_solver *s) {
+    const long nlines = s->g.ax[0].N * s->g.ax[1].N;
+    int rc = s->alloc(s->linfo, (size_t)nlines * LINE_RECORD_BYTES, "line table");
+    if (rc) return rc;
+    LineArgs<T> a = line_args<T>(s, reinterpret_cast<const T *>(s->psi[0]), reinterpret_cast<T *>(s->psi[1]), 1);
+    HIPCHK(launch_line_table<T>(a, s->linfo, s->stream), "line table");
+    HIPCHK(hipStreamSynchronize(s->stream), "line table");
+    return 0;
+}
+int build_line_table(pstd_solver *s) {
+    return (s->esz == 8) ? build_line_table_t<double>(s) : build_line_table_t<float>(s);
 }
 
 // Launch one axis pass (x: 0, y: 1 as plane passes; z: 2 fused) from psi
@@ -830,6 +851,7 @@ struct pstd_batch {
     hipStream_t stream = nullptr;
     size_t esz = 8;
     int KB = 0, block = 256;
+    bool kb_uniform[3] = {true, true, true};   // every problem has KB on that axis
     void *lists[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [parity][pass]
     int2 *maps[3] = {nullptr, nullptr, nullptr};
     int nmap[3] = {0, 0, 0};
@@ -932,12 +954,19 @@ int batch_build(pstd_batch *b) {
 template <typename T>
 int batch_step_launch(pstd_batch *b, int parity) {
     const int nprob = (int)b->sv.size();
+    // the compile-time fragment count applies when all problems share the
+    // batch's KB on that axis
+    const int KB = b->KB;
     HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][0]), b->maps[0], b->nmap[0],
-                                 b->gx[0], b->block, b->KB, b->stream), "batch x pass");
+                                 b->gx[0], b->block, KB, b->kb_uniform[0] && b->sv[0]->lay[0].KB == KB, b->stream),
+           "batch x pass");
     HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][1]), b->maps[1], b->nmap[1],
-                                 b->gx[1], b->block, b->KB, b->stream), "batch y pass");
+                                 b->gx[1], b->block, KB, b->kb_uniform[1] && b->sv[0]->lay[1].KB == KB, b->stream),
+           "batch y pass");
     HIPCHK(launch_batch_line<T>(reinterpret_cast<const LineArgs<T> *>(b->lists[parity][2]), b->maps[2], b->nmap[2],
-                                b->clock, b->step, nprob, b->gx[2], b->block, b->KB, b->stream), "batch z pass");
+                                b->clock, b->step, nprob, b->gx[2], b->block, KB,
+                                b->kb_uniform[2] && b->sv[0]->lay[2].KB == KB, b->stream),
+           "batch z pass");
     HIPCHK(launch_tick(b->step, b->stream), "batch tick");
     return 0;
 }
@@ -1013,6 +1042,7 @@ int pstd_batch_create(int n, const pstd_params *params, const double *const *vbo
                 rc = fail(PSTD_EINVAL, "batched passes need L <= 96 and cores <= 128 on every axis");
             s->method[a] = PSTD_METHOD_GEMM;
             b->KB = std::max(b->KB, s->lay[a].KB);
+            if (s->lay[a].KB != b->sv[0]->lay[a].KB) b->kb_uniform[a] = false;
             b->block = std::max(b->block, 64 * std::max(4, s->lay[a].MB));
         }
     }
