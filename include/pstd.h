@@ -178,6 +178,22 @@ int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix);
 /* Zero the accumulators (ResetSurfaceTerms, PSTD_QScat.cpp:2156-2189). */
 int pstd_reset_surface_terms(pstd_solver *s);
 
+/* Steady-state detector.  The driver accumulates the surface terms over
+ * windows of one period (IterationsToAccumulateSurfaceTerms steps) at fixed
+ * step counts (qscat.cpp:176-196); here windows run back to back: reset,
+ * pstd_update_accumulate(steps per period), then the relative L2 change of
+ * the surface psi of all six faces against the previous window,
+ * ||S_k - S_(k-1)|| / ||S_k|| (both windows hold the same number of steps).
+ * Stops after the first window k >= 1 with change <= tol, or after
+ * max_periods windows.  history (may be NULL, else max_periods doubles)
+ * receives the change of each window (1.0 for window 0).  On return the
+ * accumulators hold the last window, ready for pstd_get_surfaces /
+ * pstd_save_virtual_surfaces; *periods_out windows were run and
+ * *converged_out is 1 if the criterion was met.  No reference counterpart
+ * beyond the fixed windows of qscat.cpp. */
+int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long *periods_out, double *history,
+                             int *converged_out);
+
 /* ---- state access (tests, warm start, diagnostics) ------------------ */
 
 /* Copy psi^n (level 0) or psi^(n-1) (level 1) of the whole global grid to

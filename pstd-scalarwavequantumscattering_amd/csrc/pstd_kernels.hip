@@ -1080,6 +1080,53 @@ hipError_t launch_tick(int *step, hipStream_t s) {
 }
 bool batch_variant_ok(int MB, int KB) { return reg_variant_ok(MB, KB); }
 
+// Change of the surface psi between two accumulation windows: sums
+// |acc - prev|^2 and |acc|^2 over the psi part of the six faces into
+// out[0..1] (FP64 atomics, out zeroed by the kernel's caller), then
+// prev = acc.  One thread per surface point.
+__global__ void __launch_bounds__(256) k_surface_change(const double2 *__restrict__ acc, double2 *__restrict__ prev,
+                                                        SurfPsiMap m, double *__restrict__ out) {
+    __shared__ double red[2][4];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0, a2 = 0;
+    if (i < m.total) {
+        int f = 0;
+        while (f < 5 && i >= m.start[f + 1]) ++f;
+        const long k = i - m.start[f];
+        const double2 a = acc[m.off[f] + k], b = prev[i];
+        const double dx = a.x - b.x, dy = a.y - b.y;
+        d2 = dx * dx + dy * dy;
+        a2 = a.x * a.x + a.y * a.y;
+        prev[i] = a;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 += __shfl_down(d2, o);
+        a2 += __shfl_down(a2, o);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = d2;
+        red[1][w] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); ++j) {
+            s0 += red[0][j];
+            s1 += red[1][j];
+        }
+        atomicAdd(out, s0);
+        atomicAdd(out + 1, s1);
+    }
+}
+
+hipError_t launch_surface_change(const double2 *acc, double2 *prev, const SurfPsiMap &m, double *out,
+                                 hipStream_t s) {
+    const int blocks = (int)((m.total + 255) / 256);
+    hipLaunchKernelGGL(k_surface_change, dim3(blocks), dim3(256), 0, s, acc, prev, m, out);
+    return hipGetLastError();
+}
+
 template <typename T> hipError_t launch_line_table(const LineArgs<T> &a, void *table, hipStream_t s) {
     const int blocks = (int)((a.nlines + 255) / 256);
     hipLaunchKernelGGL(k_line_table<T>, dim3(blocks), dim3(256), 0, s, a, reinterpret_cast<LineInfo *>(table));

@@ -118,6 +118,15 @@ hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap,
                              int nprob, int grid_x, int block, int KB, bool kb_uniform, hipStream_t s);
 hipError_t launch_tick(int *step, hipStream_t s);
 constexpr size_t LINE_RECORD_BYTES = 64;
+// psi part of the six faces' accumulators: face f's plane_n points start at
+// acc + off[f]; start[f] is its first index in the packed copy (total points)
+struct SurfPsiMap {
+    long off[6];
+    long start[6];
+    long total;
+};
+hipError_t launch_surface_change(const double2 *acc, double2 *prev, const SurfPsiMap &m, double *out,
+                                 hipStream_t s);
 template <typename T> hipError_t launch_line_table(const LineArgs<T> &a, void *table, hipStream_t s);
 constexpr int FFT_NC = 16;       // lines per FFT tile
 constexpr int FFT_LMAX = 512;    // longest local line of the FFT variant

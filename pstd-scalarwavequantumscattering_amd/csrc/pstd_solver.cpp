@@ -70,6 +70,9 @@ struct pstd_solver {
     void *linfo = nullptr;        // per-line z-pass records (register variant)
     double *surf_w = nullptr;     // 6 faces x (ps, fd) x L_a
     double2 *surf_acc = nullptr;  // 6 faces x 3 x plane
+    double2 *surf_prev = nullptr; // steady-state detector: psi part of the previous window
+    double *surf_chg = nullptr;   // detector sums [|acc - prev|^2, |acc|^2]
+    SurfPsiMap psimap;
     SurfaceOp sop[6];
     long plane_off[6], plane_n[6];
     long nTau = 1, nSur = 0;
@@ -583,6 +586,48 @@ int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix) {
     unpack_body(s, PSTD_DERIV_FD, acc, body.data());
     if ((rc = write_surface_file(s, std::string(prefix) + "_FD_" + std::to_string(n0) + ".dat", body))) return rc;
     return pstd_reset_surface_terms(s);
+}
+
+int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long *periods_out, double *history,
+                             int *converged_out) {
+    if (!s || !periods_out || !converged_out || max_periods < 1 || !(tol > 0.0))
+        return fail(PSTD_EINVAL, "bad steady-state arguments");
+    *periods_out = 0;
+    *converged_out = 0;
+    if (!s->surf_prev) {   // detector buffers on first use
+        SurfPsiMap &m = s->psimap;
+        long start = 0;
+        for (int f = 0; f < 6; ++f) {
+            m.off[f] = s->plane_off[f];   // psi is the first of the face's three planes
+            m.start[f] = start;
+            start += s->plane_n[f];
+        }
+        m.total = start;
+        int rc = s->alloc(s->surf_prev, (size_t)m.total * 16, "steady-state buffer");
+        if (!rc) rc = s->alloc(s->surf_chg, 2 * sizeof(double), "steady-state buffer");
+        if (rc) return rc;
+    }
+    const long nper = pstd_steps_per_period(s);
+    int rc = pstd_reset_surface_terms(s);
+    if (rc) return rc;
+    bool conv = false;
+    long k = 0;
+    for (; k < max_periods && !conv; ++k) {
+        if (k > 0 && (rc = pstd_reset_surface_terms(s))) return rc;
+        if ((rc = pstd_update_accumulate(s, nper))) return rc;
+        // the detector runs eagerly on the solver's stream (never captured)
+        HIPCHK(hipMemsetAsync(s->surf_chg, 0, 2 * sizeof(double), s->stream), "steady-state sums");
+        HIPCHK(launch_surface_change(s->surf_acc, s->surf_prev, s->psimap, s->surf_chg, s->stream), "steady state");
+        double sums[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(sums, s->surf_chg, sizeof(sums), hipMemcpyDeviceToHost, s->stream), "steady state");
+        HIPCHK(hipStreamSynchronize(s->stream), "steady state");
+        const double change = (k > 0 && sums[1] > 0.0) ? std::sqrt(sums[0] / sums[1]) : 1.0;
+        if (history) history[k] = change;
+        if (k > 0 && change <= tol) conv = true;
+    }
+    *periods_out = k;
+    *converged_out = conv ? 1 : 0;
+    return 0;
 }
 
 int pstd_reset_surface_terms(pstd_solver *s) {

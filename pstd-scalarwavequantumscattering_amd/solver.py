@@ -25,7 +25,7 @@ EXPORTS = [
     "pstd_time_index", "pstd_get_surfaces", "pstd_save_virtual_surfaces",
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
-    "pstd_set_method", "pstd_get_method", "pstd_autotune",
+    "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
 ]
@@ -88,6 +88,8 @@ def library():
         L.pstd_get_method.argtypes = [vp, i]
         L.pstd_autotune.restype = i
         L.pstd_autotune.argtypes = [vp, C.c_char_p, i, dp]
+        L.pstd_run_to_steady_state.restype = i
+        L.pstd_run_to_steady_state.argtypes = [vp, C.c_double, l, C.POINTER(l), dp, C.POINTER(i)]
         L.pstd_batch_create.restype = i
         L.pstd_batch_create.argtypes = [i, C.POINTER(PstdParams), C.POINTER(dp), C.POINTER(sz), C.POINTER(vp)]
         for name in ("pstd_batch_update", "pstd_batch_update_accumulate"):
@@ -235,6 +237,18 @@ class PSTDQScat:
 
     def reset_surface_terms(self):
         _check(library().pstd_reset_surface_terms(self.h))
+
+    def run_to_steady_state(self, tol=1e-6, max_periods=100):
+        """Back-to-back one-period surface windows until the relative change
+        of surface psi between windows is <= tol (pstd_run_to_steady_state).
+        Returns (converged, periods, steps, history); the accumulators hold
+        the last window."""
+        hist = np.zeros(int(max_periods))
+        periods, conv = C.c_long(0), C.c_int(0)
+        _check(library().pstd_run_to_steady_state(self.h, float(tol), int(max_periods), C.byref(periods),
+                                                  _dp(hist), C.byref(conv)))
+        n = periods.value
+        return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy()
 
     def get_surfaces(self, deriv=PSTD_DERIV_PS):
         n = surface_body_len(self.N)
