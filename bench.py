@@ -14,6 +14,11 @@ One step = one Update of the whole grid (three windowed local-Fourier
 Laplacian passes with the potential, TF/SF source and absorber fused into
 the last).  value = grid points x steps / seconds, inputs resident in HBM.
 
+Extras on the same line: the FP32 path (FP64 phase) on grf224 and on the
+320^3 wells workload, the 48-problem batch, and time-to-steady-state of the
+FP32 grf224 solve (first one-period surface window whose change against the
+previous window is below each tolerance) with its error against FP64.
+
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs one
 GPU; the multi-GPU decomposition is a design only.
 """
@@ -58,6 +63,57 @@ def gpu_run(pkg, W, precision, steps, warmup, fn):
     return p, npts, dt, ms, acc, tune
 
 
+def steady_state(pkg, W, max_periods, err_steps):
+    """Time-to-steady-state of the FP32 grf224 solve: back-to-back one-period
+    surface windows (pstd_run_to_steady_state, tol 1e-6, capped), the first
+    window at which each tolerance is met, and the FP32 error against the FP64
+    path (parity-equal to the FP64 oracle) after err_steps steps."""
+    import numpy as np
+    p, V = W.config_grf224(precision=pkg.PSTD_FP32)
+    s = pkg.PSTDQScat(p, V)
+    nper = s.IterationsToAccumulateSurfaceTerms()
+    s.synchronize()
+    t0 = time.perf_counter()
+    conv, periods, steps, hist = s.run_to_steady_state(tol=1e-6, max_periods=max_periods)
+    dt = time.perf_counter() - t0
+    s.close()
+    per = dt / periods
+    reach = {}
+    for tol in (1e-2, 1e-3, 1e-4, 1e-6):
+        k = next((i for i in range(1, len(hist)) if hist[i] <= tol), None)
+        reach[f"{tol:g}"] = None if k is None else {"periods": k + 1, "steps": (k + 1) * nper,
+                                                     "seconds": round((k + 1) * per, 3)}
+    a = pkg.PSTDQScat(*W.config_grf224(precision=pkg.PSTD_FP32))
+    b = pkg.PSTDQScat(*W.config_grf224(precision=pkg.PSTD_FP64))
+    a.Update(err_steps)
+    b.Update(err_steps)
+    x, y = a.get_psi(0), b.get_psi(0)
+    a.close()
+    b.close()
+    return {"workload": "grf224 FP32 (FP64 phase)", "criterion": "relative L2 change of surface psi between "
+            "consecutive one-period DFT windows", "steps_per_period": nper, "periods_run": periods,
+            "converged_1e-6": conv, "last_change": float(hist[-1]), "seconds_per_period": round(per, 4),
+            "reached": reach,
+            "fp32_psi_rel_l2_vs_fp64": {"steps": err_steps, "value": float(np.linalg.norm(x - y) / np.linalg.norm(y))}}
+
+
+def batch_run(pkg, W, steps):
+    """48 uneven FP32 problems (grids 64..160^3, L = 64) stepped as one batch:
+    one launch per pass over all problems, steps replayed as a captured graph."""
+    probs = W.config_batch48()
+    b = pkg.PSTDBatch([q for q, _ in probs], [v for _, v in probs])
+    npts = sum(int(m.N[0]) * int(m.N[1]) * int(m.N[2]) for m in b.members)
+    b.Update(4)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.Update(steps)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    b.close()
+    return {"workload": "batch48: 48 problems, N in {64, 96, 128, 160}^3, L = 64, h = 16, FP32 (FP64 phase)",
+            "grid_points": npts, "steps": steps, "ms_per_step": dt / steps * 1e3, "value": npts * steps / dt}
+
+
 def cpu_baseline(W, steps):
     """The CPU oracle (FP64 restatement of the reference algorithm, OpenMP
     over lines) on the same workload, a bounded number of steps."""
@@ -98,6 +154,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--steady-periods", type=int, default=60)
+    ap.add_argument("--no-steady", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -148,6 +206,10 @@ def main():
                            "kernel_ms": [float(v) for v in ms2[:3]],
                            "line_operator": ["fft" if m else "gemm" for m in t2["methods"]]}
         line["fp32_fp64phase"] = extra
+    if not args.no_fp32:
+        line["batch48_fp32"] = batch_run(pkg, W, 100)
+    if not args.no_steady:
+        line["time_to_steady_state"] = steady_state(pkg, W, args.steady_periods, 4000)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
     print(json.dumps(line), flush=True)
