@@ -1,0 +1,682 @@
+// Warm-start network for gfx950: 3D convolutional encoder-decoder with
+// bfloat16 weights and activations on MFMA (FP32 accumulation), hand-written
+// forward, backward and Adam kernels, and the host driver behind
+// include/pstd_cnn.h.  No reference counterpart (the reference has no
+// learned component).
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16: a wave owns
+// 16 consecutive voxels and every output channel; for each of the 27 taps
+// and each 16-channel input block, lane l supplies A[voxel l%16][4(l/16)..+3]
+// (4 contiguous channels, 8 bytes) and B[4(l/16)..+3][cout l%16] from
+// W[tap][cout][cin] (cin contiguous).  Result element q of lane l is
+// D[voxel 4(l/16)+q][cout l%16] (layout measured on gfx950 with an
+// in-bounds probe).  The backward-data pass is the same kernel on the
+// pre-activation gradient with the kernel flipped and transposed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pstd.h"
+#include "../../include/pstd_cnn.h"
+
+namespace pstd {
+int report_error(int code, const std::string &m);
+int report_hip_error(hipError_t e, const char *what);
+}  // namespace pstd
+
+namespace {
+
+using u16 = unsigned short;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr float SLOPE = 0.1f;
+
+#define CHK(call, what)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return pstd::report_hip_error(e_, what);    \
+    } while (0)
+
+// float -> bfloat16, round to nearest even (inputs are finite)
+__device__ __forceinline__ u16 f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+
+struct ConvShape {
+    int B, R;          // batch, grid edge (R^3 voxels per sample)
+    int Cin, Cout;     // multiples of 16, Cout <= 128
+    int act;           // leaky ReLU on the output (mode 0)
+};
+
+__device__ __forceinline__ void voxel_coords(long v, int R, long &n, int &z, int &y, int &x) {
+    x = (int)(v % R);
+    long q = v / R;
+    y = (int)(q % R);
+    q /= R;
+    z = (int)(q % R);
+    n = q / R;
+}
+
+// MODE 0: bf16 out = act(conv + bias); MODE 1: fp32 out = conv + bias;
+// MODE 2: fp32 out += conv (backward data into a gradient buffer).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv(const u16 *__restrict__ x, const u16 *__restrict__ w,
+                                              const float *__restrict__ bias, void *__restrict__ out, ConvShape s) {
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int R = s.R;
+    const long V = (long)s.B * R * R * R;
+    const long v0 = wave * 16;
+    if (v0 >= V) return;
+    const long va = v0 + (lane & 15);
+    const bool va_ok = va < V;
+    long an = 0;
+    int az = 0, ay = 0, ax = 0;
+    if (va_ok) voxel_coords(va, R, an, az, ay, ax);
+    const int kq = lane >> 4;
+    const int NB = s.Cout >> 4, CB = s.Cin >> 4;
+    v4f acc[8];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) acc[nb] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 27; ++t) {
+        const int zz = az + t / 9 - 1, yy = ay + (t / 3) % 3 - 1, xx = ax + t % 3 - 1;
+        const bool ok = va_ok && zz >= 0 && zz < R && yy >= 0 && yy < R && xx >= 0 && xx < R;
+        const long vs = ok ? ((an * R + zz) * R + yy) * R + xx : 0;
+        const u16 *xa = x + vs * s.Cin + 4 * kq;
+        const u16 *wb = w + (long)t * s.Cout * s.Cin + (long)(lane & 15) * s.Cin + 4 * kq;
+        for (int cb = 0; cb < CB; ++cb) {
+            v4s a = {0, 0, 0, 0};
+            if (ok) a = *reinterpret_cast<const v4s *>(xa + cb * 16);
+#pragma unroll
+            for (int nb = 0; nb < 8; ++nb) {
+                if (nb < NB) {
+                    const v4s b = *reinterpret_cast<const v4s *>(wb + (long)nb * 16 * s.Cin + cb * 16);
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+        if (nb >= NB) break;
+        const int co = nb * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long v = v0 + 4 * kq + q;
+            if (v >= V) continue;
+            float val = acc[nb][q];
+            if (MODE == 0) {
+                val += bias[co];
+                if (s.act) val = val > 0.f ? val : SLOPE * val;
+                reinterpret_cast<u16 *>(out)[v * s.Cout + co] = f2bf(val);
+            } else if (MODE == 1) {
+                reinterpret_cast<float *>(out)[v * s.Cout + co] = val + bias[co];
+            } else {
+                reinterpret_cast<float *>(out)[v * s.Cout + co] += val;
+            }
+        }
+    }
+}
+
+// dW[t][co][ci] += sum_v dz[v][co] x[v + tap t][ci] over one chunk of voxels
+// per block: grid (chunks, 27, (Cout/16) (Cin/16)), 4 waves split the chunk
+// in 16-voxel k steps; the 16 x 16 tile is reduced in LDS and added with
+// FP32 atomics.
+__global__ void __launch_bounds__(256) k_conv_bwd_w(const u16 *__restrict__ x, const u16 *__restrict__ dz,
+                                                    float *__restrict__ dW, ConvShape s, long chunk) {
+    __shared__ float red[4][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kq = lane >> 4;
+    const int t = blockIdx.y;
+    const int CB = s.Cin >> 4;
+    const int cob = blockIdx.z / CB, cib = blockIdx.z % CB;
+    const int R = s.R;
+    const long V = (long)s.B * R * R * R;
+    const long c0 = (long)blockIdx.x * chunk, c1 = (c0 + chunk < V) ? c0 + chunk : V;
+    const int dz_ = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    for (long vb = c0 + 16 * w; vb < c1; vb += 64) {
+        v4s a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long v = vb + 4 * kq + j;
+            if (v >= c1) continue;
+            a[j] = (short)dz[v * s.Cout + cob * 16 + (lane & 15)];
+            long n;
+            int z, y, xx;
+            voxel_coords(v, R, n, z, y, xx);
+            z += dz_;
+            y += dy;
+            xx += dx;
+            if (z >= 0 && z < R && y >= 0 && y < R && xx >= 0 && xx < R)
+                b[j] = (short)x[(((n * R + z) * R + y) * R + xx) * s.Cin + cib * 16 + (lane & 15)];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[w][lane * 4 + q] = acc[q];
+    __syncthreads();
+    const int i = threadIdx.x;
+    const float sum = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+    const int ln = i >> 2, q = i & 3;
+    const int co = cob * 16 + 4 * (ln >> 4) + q, ci = cib * 16 + (ln & 15);
+    atomicAdd(&dW[((long)t * s.Cout + co) * s.Cin + ci], sum);
+}
+
+// db[c] += sum over voxels of dz[v][c]; the grid's thread count is a
+// multiple of C, so every thread keeps one channel.
+__global__ void __launch_bounds__(256) k_bias_grad(const u16 *__restrict__ dz, float *__restrict__ db, long n, int C) {
+    const long T = (long)gridDim.x * blockDim.x;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float sum = 0.f;
+    for (long i = i0; i < n; i += T) sum += bf2f(dz[i]);
+    atomicAdd(&db[i0 % C], sum);
+}
+
+// 2x average pooling: y (edge Ro) from x (edge 2 Ro)
+__global__ void k_pool_fwd(const u16 *__restrict__ x, u16 *__restrict__ y, int B, int Ro, int C) {
+    const long n = (long)B * Ro * Ro * Ro * C;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long v = i / C, nb;
+        int z, yy, xx;
+        voxel_coords(v, Ro, nb, z, yy, xx);
+        const int Ri = 2 * Ro;
+        float s = 0.f;
+        for (int k = 0; k < 8; ++k) {
+            const long vi = ((nb * Ri + 2 * z + (k >> 2)) * Ri + 2 * yy + ((k >> 1) & 1)) * Ri + 2 * xx + (k & 1);
+            s += bf2f(x[vi * C + c]);
+        }
+        y[i] = f2bf(s * 0.125f);
+    }
+}
+// dx (edge 2 Ro) += dy[parent] / 8
+__global__ void k_pool_bwd(const float *__restrict__ dy, float *__restrict__ dx, int B, int Ro, int C) {
+    const int Ri = 2 * Ro;
+    const long n = (long)B * Ri * Ri * Ri * C;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long v = i / C, nb;
+        int z, yy, xx;
+        voxel_coords(v, Ri, nb, z, yy, xx);
+        const long vp = ((nb * Ro + z / 2) * Ro + yy / 2) * Ro + xx / 2;
+        dx[i] += dy[vp * C + c] * 0.125f;
+    }
+}
+// 2x nearest upsampling: y (edge 2 Ri) from x (edge Ri)
+__global__ void k_up_fwd(const u16 *__restrict__ x, u16 *__restrict__ y, int B, int Ri, int C) {
+    const int Ro = 2 * Ri;
+    const long n = (long)B * Ro * Ro * Ro * C;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long v = i / C, nb;
+        int z, yy, xx;
+        voxel_coords(v, Ro, nb, z, yy, xx);
+        y[i] = x[(((nb * Ri + z / 2) * Ri + yy / 2) * Ri + xx / 2) * C + c];
+    }
+}
+// dx (edge Ri) += sum of the 8 children of dy (edge 2 Ri)
+__global__ void k_up_bwd(const float *__restrict__ dy, float *__restrict__ dx, int B, int Ri, int C) {
+    const long n = (long)B * Ri * Ri * Ri * C;
+    const int Ro = 2 * Ri;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long v = i / C, nb;
+        int z, yy, xx;
+        voxel_coords(v, Ri, nb, z, yy, xx);
+        float s = 0.f;
+        for (int k = 0; k < 8; ++k) {
+            const long vo = ((nb * Ro + 2 * z + (k >> 2)) * Ro + 2 * yy + ((k >> 1) & 1)) * Ro + 2 * xx + (k & 1);
+            s += dy[vo * C + c];
+        }
+        dx[i] += s;
+    }
+}
+__global__ void k_add_fwd(const u16 *__restrict__ a, const u16 *__restrict__ b, u16 *__restrict__ s, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+__global__ void k_accum(const float *__restrict__ g, float *__restrict__ a, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] += g[i];
+}
+// pre-activation gradient, rounded to bf16 for the matrix products
+__global__ void k_act_bwd(const float *__restrict__ g, const u16 *__restrict__ a, u16 *__restrict__ dz, long n,
+                          int act) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        float d = g[i];
+        if (act && !(bf2f(a[i]) > 0.f)) d *= SLOPE;
+        dz[i] = f2bf(d);
+    }
+}
+__global__ void k_to_bf16(const float *__restrict__ f, u16 *__restrict__ h, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        h[i] = f2bf(f[i]);
+}
+// wt[26 - t][ci][co] = w[t][co][ci]
+__global__ void k_flip_transpose(const u16 *__restrict__ w, u16 *__restrict__ wt, int Cout, int Cin) {
+    const long n = 27L * Cout * Cin;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % Cin);
+        const long r = i / Cin;
+        const int co = (int)(r % Cout), t = (int)(r / Cout);
+        wt[((long)(26 - t) * Cin + ci) * Cout + co] = w[i];
+    }
+}
+// loss = sum over channels 0..1 of (out - target)^2 (divided on the host),
+// gout[v][c] = 2 (out - target) / count for c < 2, else 0
+__global__ void __launch_bounds__(256) k_mse(const float *__restrict__ out, const float *__restrict__ target,
+                                             float *__restrict__ gout, float *__restrict__ loss, long V, int C,
+                                             float inv_count) {
+    __shared__ float red[4];
+    float s = 0.f;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < V * C; i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        float g = 0.f;
+        if (c < PSTD_CNN_OUT_CHANNELS) {
+            const float d = out[i] - target[(i / C) * PSTD_CNN_OUT_CHANNELS + c];
+            s += d * d;
+            g = 2.f * d * inv_count;
+        }
+        gout[i] = g;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
+}
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, u16 *__restrict__ pb, long n, float lr, float c1, float c2) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = 0.9f * m[i] + 0.1f * gi;
+        const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float np = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+        p[i] = np;
+        pb[i] = f2bf(np);
+    }
+}
+
+int grid_for(long n) {
+    long g = (n + 255) / 256;
+    return (int)(g < 4096 ? (g > 0 ? g : 1) : 4096);
+}
+
+enum OpKind { CONV, POOL, UP, ADD };
+struct OpDef {
+    OpKind k;
+    const char *in, *in2, *out;
+    int ci, co;
+    bool act;
+};
+// layer list; the CPU oracle (oracle/cnn_oracle.py) holds the same list
+const OpDef OPS[] = {
+    {CONV, "x0", nullptr, "e1", 16, 16, true},     {CONV, "e1", nullptr, "e1b", 16, 16, true},
+    {POOL, "e1b", nullptr, "p1", 0, 0, false},     {CONV, "p1", nullptr, "e2", 16, 32, true},
+    {CONV, "e2", nullptr, "e2b", 32, 32, true},    {POOL, "e2b", nullptr, "p2", 0, 0, false},
+    {CONV, "p2", nullptr, "e3", 32, 64, true},     {CONV, "e3", nullptr, "e3b", 64, 64, true},
+    {POOL, "e3b", nullptr, "p3", 0, 0, false},     {CONV, "p3", nullptr, "b1", 64, 128, true},
+    {CONV, "b1", nullptr, "b2", 128, 128, true},   {CONV, "b2", nullptr, "b3", 128, 128, true},
+    {UP, "b3", nullptr, "u3", 0, 0, false},        {CONV, "u3", nullptr, "d3", 128, 64, true},
+    {ADD, "d3", "e3b", "s3", 0, 0, false},         {CONV, "s3", nullptr, "d3b", 64, 64, true},
+    {UP, "d3b", nullptr, "u2", 0, 0, false},       {CONV, "u2", nullptr, "d2", 64, 32, true},
+    {ADD, "d2", "e2b", "s2", 0, 0, false},         {CONV, "s2", nullptr, "d2b", 32, 32, true},
+    {UP, "d2b", nullptr, "u1", 0, 0, false},       {CONV, "u1", nullptr, "d1", 32, 16, true},
+    {ADD, "d1", "e1b", "s1", 0, 0, false},         {CONV, "s1", nullptr, "d1b", 16, 16, true},
+    {CONV, "d1b", nullptr, "out", 16, 16, false},
+};
+
+struct Tensor {
+    int level, C;
+    long V;            // voxels (batch included)
+    u16 *a = nullptr;  // activation (bf16)
+    float *g = nullptr;
+};
+struct Op {
+    OpKind k;
+    int in, in2, out, ci, co;
+    bool act;
+    long woff, boff;
+};
+
+}  // namespace
+
+struct pstd_cnn {
+    int R = 0, B = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Tensor> T;
+    std::vector<Op> ops;
+    long np = 0;
+    float *P = nullptr, *G = nullptr, *M = nullptr, *Vm = nullptr;
+    u16 *Pb = nullptr;
+    u16 *abuf = nullptr;
+    float *gbuf = nullptr;
+    size_t gbytes = 0;
+    u16 *dz = nullptr, *wt = nullptr;
+    float *in32 = nullptr, *out32 = nullptr, *target = nullptr, *loss = nullptr;
+    long step = 0;
+    std::vector<void *> allocs;
+    ~pstd_cnn() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *q : allocs) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename X>
+    int alloc(X *&p, size_t bytes, const char *what) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+        if (e != hipSuccess) return pstd::report_hip_error(e, what);
+        allocs.push_back(q);
+        p = reinterpret_cast<X *>(q);
+        return 0;
+    }
+};
+
+namespace {
+
+long count_params() {
+    long n = 0;
+    for (const OpDef &d : OPS)
+        if (d.k == CONV) n += 27L * d.ci * d.co + d.co;
+    return n;
+}
+
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int launch_conv(pstd_cnn *n, int mode, const u16 *x, const u16 *w, const float *b, void *out, int level, int ci,
+                int co, bool act) {
+    ConvShape s{n->B, n->R >> level, ci, co, act ? 1 : 0};
+    const long V = (long)s.B * s.R * s.R * s.R;
+    const long waves = (V + 15) / 16;
+    const int blocks = (int)((waves + 3) / 4);
+    if (mode == 0) hipLaunchKernelGGL(k_conv<0>, dim3(blocks), dim3(256), 0, n->stream, x, w, b, out, s);
+    else if (mode == 1) hipLaunchKernelGGL(k_conv<1>, dim3(blocks), dim3(256), 0, n->stream, x, w, b, out, s);
+    else hipLaunchKernelGGL(k_conv<2>, dim3(blocks), dim3(256), 0, n->stream, x, w, b, out, s);
+    CHK(hipGetLastError(), "conv");
+    return 0;
+}
+
+int forward_impl(pstd_cnn *n, const float *input) {
+    const long V0 = n->T[0].V;
+    CHK(hipMemcpyAsync(n->in32, input, V0 * PSTD_CNN_IN_CHANNELS * 4, hipMemcpyHostToDevice, n->stream), "input");
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(V0 * 16)), dim3(256), 0, n->stream, n->in32, n->T[0].a,
+                       V0 * PSTD_CNN_IN_CHANNELS);
+    for (const Op &o : n->ops) {
+        Tensor &ti = n->T[o.in], &to = n->T[o.out];
+        const long ne = to.V * to.C;
+        switch (o.k) {
+        case CONV: {
+            const bool last = (&o == &n->ops.back());
+            int rc = launch_conv(n, last ? 1 : 0, ti.a, n->Pb + o.woff, n->P + o.boff, last ? (void *)n->out32 : to.a,
+                                 ti.level, o.ci, o.co, o.act);
+            if (rc) return rc;
+            break;
+        }
+        case POOL:
+            hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(ne)), dim3(256), 0, n->stream, ti.a, to.a, n->B,
+                               n->R >> to.level, to.C);
+            break;
+        case UP:
+            hipLaunchKernelGGL(k_up_fwd, dim3(grid_for(ne)), dim3(256), 0, n->stream, ti.a, to.a, n->B,
+                               n->R >> ti.level, to.C);
+            break;
+        case ADD:
+            hipLaunchKernelGGL(k_add_fwd, dim3(grid_for(ne)), dim3(256), 0, n->stream, ti.a, n->T[o.in2].a, to.a, ne);
+            break;
+        }
+        CHK(hipGetLastError(), "forward");
+    }
+    return 0;
+}
+
+int backward_impl(pstd_cnn *n, const float *target, float *loss_out) {
+    const Tensor &out = n->T[n->ops.back().out];
+    const long V0 = out.V;
+    CHK(hipMemcpyAsync(n->target, target, V0 * PSTD_CNN_OUT_CHANNELS * 4, hipMemcpyHostToDevice, n->stream), "target");
+    CHK(hipMemsetAsync(n->gbuf, 0, n->gbytes, n->stream), "grad zero");
+    CHK(hipMemsetAsync(n->G, 0, n->np * 4, n->stream), "grad zero");
+    CHK(hipMemsetAsync(n->loss, 0, 4, n->stream), "loss zero");
+    const float inv = 1.f / (float)(V0 * PSTD_CNN_OUT_CHANNELS);
+    hipLaunchKernelGGL(k_mse, dim3(grid_for(V0 * out.C)), dim3(256), 0, n->stream, n->out32, n->target, out.g, n->loss,
+                       V0, out.C, inv);
+    CHK(hipGetLastError(), "loss");
+    for (int k = (int)n->ops.size() - 1; k >= 0; --k) {
+        const Op &o = n->ops[k];
+        Tensor &ti = n->T[o.in], &to = n->T[o.out];
+        const long ne = to.V * to.C;
+        switch (o.k) {
+        case CONV: {
+            // pre-activation gradient (bf16), weight and bias gradients
+            hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(ne)), dim3(256), 0, n->stream, to.g, to.a, n->dz, ne,
+                               o.act ? 1 : 0);
+            ConvShape s{n->B, n->R >> ti.level, o.ci, o.co, 0};
+            const long chunk = 4096;
+            dim3 gw((unsigned)((ti.V + chunk - 1) / chunk), 27, (unsigned)((o.co / 16) * (o.ci / 16)));
+            hipLaunchKernelGGL(k_conv_bwd_w, gw, dim3(256), 0, n->stream, ti.a, n->dz, n->G + o.woff, s, chunk);
+            // 256 threads per block: the thread count is a multiple of the channel count
+            hipLaunchKernelGGL(k_bias_grad, dim3(grid_for(ne)), dim3(256), 0, n->stream, n->dz, n->G + o.boff, ne, o.co);
+            if (k > 0) {   // the input image needs no gradient
+                hipLaunchKernelGGL(k_flip_transpose, dim3(grid_for(27L * o.co * o.ci)), dim3(256), 0, n->stream,
+                                   n->Pb + o.woff, n->wt, o.co, o.ci);
+                int rc = launch_conv(n, 2, n->dz, n->wt, nullptr, ti.g, ti.level, o.co, o.ci, false);
+                if (rc) return rc;
+            }
+            break;
+        }
+        case POOL:
+            hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(ti.V * ti.C)), dim3(256), 0, n->stream, to.g, ti.g, n->B,
+                               n->R >> to.level, to.C);
+            break;
+        case UP:
+            hipLaunchKernelGGL(k_up_bwd, dim3(grid_for(ti.V * ti.C)), dim3(256), 0, n->stream, to.g, ti.g, n->B,
+                               n->R >> ti.level, to.C);
+            break;
+        case ADD:
+            hipLaunchKernelGGL(k_accum, dim3(grid_for(ne)), dim3(256), 0, n->stream, to.g, ti.g, ne);
+            hipLaunchKernelGGL(k_accum, dim3(grid_for(ne)), dim3(256), 0, n->stream, to.g, n->T[o.in2].g, ne);
+            break;
+        }
+        CHK(hipGetLastError(), "backward");
+    }
+    if (loss_out) {
+        float l = 0.f;
+        CHK(hipMemcpyAsync(&l, n->loss, 4, hipMemcpyDeviceToHost, n->stream), "loss");
+        CHK(hipStreamSynchronize(n->stream), "loss");
+        *loss_out = l * inv;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+long pstd_cnn_num_params(void) { return count_params(); }
+
+int pstd_cnn_create(int resolution, int batch, unsigned long long seed, pstd_cnn **out) {
+    if (!out || resolution < 8 || resolution % 8 || batch < 1)
+        return pstd::report_error(PSTD_EINVAL, "resolution must be a positive multiple of 8, batch >= 1");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return pstd::report_error(PSTD_ENODEV, "no HIP device");
+    int dev = 0;
+    hipDeviceProp_t prop;
+    CHK(hipGetDevice(&dev), "hipGetDevice");
+    CHK(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return pstd::report_error(PSTD_ENODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    pstd_cnn *n = new (std::nothrow) pstd_cnn();
+    if (!n) return pstd::report_error(PSTD_ENOMEM, "host allocation");
+    n->R = resolution;
+    n->B = batch;
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete n;
+        return pstd::report_error(PSTD_ENODEV, "hipStreamCreate");
+    }
+    // tensors and ops
+    std::map<std::string, int> id;
+    auto add_tensor = [&](const std::string &name, int level, int C) {
+        Tensor t;
+        t.level = level;
+        t.C = C;
+        const long r = resolution >> level;
+        t.V = (long)batch * r * r * r;
+        id[name] = (int)n->T.size();
+        n->T.push_back(t);
+    };
+    add_tensor("x0", 0, PSTD_CNN_IN_CHANNELS);
+    long off = 0;
+    for (const OpDef &d : OPS) {
+        const Tensor &ti = n->T[id[d.in]];
+        Op o{d.k, id[d.in], d.in2 ? id[d.in2] : -1, -1, d.ci, d.co, d.act, 0, 0};
+        int level = ti.level, C = ti.C;
+        if (d.k == CONV) {
+            C = d.co;
+            o.woff = off;
+            off += 27L * d.ci * d.co;
+            o.boff = off;
+            off += d.co;
+        } else if (d.k == POOL) {
+            level += 1;
+        } else if (d.k == UP) {
+            level -= 1;
+        }
+        add_tensor(d.out, level, C);
+        o.out = id[d.out];
+        n->ops.push_back(o);
+    }
+    n->np = off;
+    size_t abytes = 0, gbytes = 0, maxe = 0;
+    for (const Tensor &t : n->T) {
+        abytes += (size_t)t.V * t.C * 2;
+        gbytes += (size_t)t.V * t.C * 4;
+        maxe = std::max(maxe, (size_t)(t.V * t.C));
+    }
+    const long V0 = n->T[0].V;
+    int rc = n->alloc(n->abuf, abytes, "activations");
+    if (!rc) rc = n->alloc(n->gbuf, gbytes, "gradients");
+    if (!rc) rc = n->alloc(n->P, n->np * 4, "parameters");
+    if (!rc) rc = n->alloc(n->G, n->np * 4, "parameters");
+    if (!rc) rc = n->alloc(n->M, n->np * 4, "parameters");
+    if (!rc) rc = n->alloc(n->Vm, n->np * 4, "parameters");
+    if (!rc) rc = n->alloc(n->Pb, n->np * 2, "parameters");
+    if (!rc) rc = n->alloc(n->dz, maxe * 2, "scratch");
+    if (!rc) rc = n->alloc(n->wt, 27L * 128 * 128 * 2, "scratch");
+    if (!rc) rc = n->alloc(n->in32, V0 * PSTD_CNN_IN_CHANNELS * 4, "input");
+    if (!rc) rc = n->alloc(n->out32, V0 * 16 * 4, "output");
+    if (!rc) rc = n->alloc(n->target, V0 * PSTD_CNN_OUT_CHANNELS * 4, "target");
+    if (!rc) rc = n->alloc(n->loss, 16, "loss");
+    if (rc) {
+        delete n;
+        return rc;
+    }
+    n->gbytes = gbytes;
+    size_t ao = 0, go = 0;
+    for (Tensor &t : n->T) {
+        t.a = n->abuf + ao;
+        t.g = n->gbuf + go;
+        ao += (size_t)t.V * t.C;
+        go += (size_t)t.V * t.C;
+    }
+    // He-normal weights (std sqrt(2 / (27 cin)), the output layer sqrt(1 / (27 cin))), zero biases
+    std::vector<float> h(n->np, 0.f);
+    uint64_t st = seed;
+    for (const Op &o : n->ops) {
+        if (o.k != CONV) continue;
+        const double sd = std::sqrt((o.act ? 2.0 : 1.0) / (27.0 * o.ci));
+        for (long i = 0; i < 27L * o.ci * o.co; ++i) {
+            const double u1 = ((splitmix(st) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+            const double u2 = (splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+            h[o.woff + i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+        }
+    }
+    rc = pstd_cnn_set_params(n, h.data(), h.size());
+    if (rc) {
+        delete n;
+        return rc;
+    }
+    *out = n;
+    return 0;
+}
+
+void pstd_cnn_destroy(pstd_cnn *net) { delete net; }
+
+int pstd_cnn_get_params(pstd_cnn *n, float *out, size_t cnt) {
+    if (!n || !out || (long)cnt != n->np) return pstd::report_error(PSTD_EINVAL, "parameter buffer size mismatch");
+    CHK(hipStreamSynchronize(n->stream), "sync");
+    CHK(hipMemcpy(out, n->P, cnt * 4, hipMemcpyDeviceToHost), "get params");
+    return 0;
+}
+
+int pstd_cnn_set_params(pstd_cnn *n, const float *in, size_t cnt) {
+    if (!n || !in || (long)cnt != n->np) return pstd::report_error(PSTD_EINVAL, "parameter buffer size mismatch");
+    CHK(hipStreamSynchronize(n->stream), "sync");
+    CHK(hipMemcpy(n->P, in, cnt * 4, hipMemcpyHostToDevice), "set params");
+    CHK(hipMemset(n->M, 0, cnt * 4), "adam state");
+    CHK(hipMemset(n->Vm, 0, cnt * 4), "adam state");
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for((long)cnt)), dim3(256), 0, n->stream, n->P, n->Pb, (long)cnt);
+    CHK(hipGetLastError(), "set params");
+    CHK(hipStreamSynchronize(n->stream), "set params");
+    n->step = 0;
+    return 0;
+}
+
+int pstd_cnn_forward(pstd_cnn *n, const float *input, float *output) {
+    if (!n || !input || !output) return pstd::report_error(PSTD_EINVAL, "null argument");
+    int rc = forward_impl(n, input);
+    if (rc) return rc;
+    // channels 0..1 of the 16 output channels
+    const long V0 = n->T[0].V;
+    std::vector<float> full(V0 * 16);
+    CHK(hipMemcpyAsync(full.data(), n->out32, full.size() * 4, hipMemcpyDeviceToHost, n->stream), "output");
+    CHK(hipStreamSynchronize(n->stream), "output");
+    for (long v = 0; v < V0; ++v) {
+        output[2 * v] = full[16 * v];
+        output[2 * v + 1] = full[16 * v + 1];
+    }
+    return 0;
+}
+
+int pstd_cnn_gradients(pstd_cnn *n, const float *input, const float *target, float *grad, float *loss) {
+    if (!n || !input || !target) return pstd::report_error(PSTD_EINVAL, "null argument");
+    int rc = forward_impl(n, input);
+    if (!rc) rc = backward_impl(n, target, loss);
+    if (rc) return rc;
+    if (grad) {
+        CHK(hipStreamSynchronize(n->stream), "sync");
+        CHK(hipMemcpy(grad, n->G, n->np * 4, hipMemcpyDeviceToHost), "gradients");
+    }
+    return 0;
+}
+
+int pstd_cnn_train_step(pstd_cnn *n, const float *input, const float *target, float lr, float *loss) {
+    if (!n || !input || !target || !(lr > 0.f)) return pstd::report_error(PSTD_EINVAL, "bad argument");
+    int rc = forward_impl(n, input);
+    if (!rc) rc = backward_impl(n, target, loss);
+    if (rc) return rc;
+    n->step += 1;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)n->step));
+    const float c2 = (float)(1.0 - std::pow(0.999, (double)n->step));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n->np)), dim3(256), 0, n->stream, n->P, n->G, n->M, n->Vm, n->Pb, n->np, lr,
+                       c1, c2);
+    CHK(hipGetLastError(), "adam");
+    if (loss) CHK(hipStreamSynchronize(n->stream), "sync");
+    return 0;
+}
+
+}  // extern "C"

@@ -39,6 +39,11 @@ int hipfail(hipError_t e, const char *what) {
 
 }  // namespace
 
+namespace pstd {
+int report_error(int code, const std::string &m) { return fail(code, m); }
+int report_hip_error(hipError_t e, const char *what) { return hipfail(e, what); }
+}  // namespace pstd
+
 struct pstd_solver {
     pstd_params p;
     Geometry g;

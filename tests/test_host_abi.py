@@ -20,7 +20,7 @@ GOLD = Path(__file__).resolve().parent / "golden"
 
 
 def test_library_exports_every_declared_symbol():
-    header = (ROOT / "include" / "pstd.h").read_text()
+    header = "".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     names = set(re.findall(r"\b(pstd_[a-z_0-9]+)\s*\(", header))
     names -= {"pstd_potential_fn"}
     lib = pstd_amd.library()
