@@ -152,9 +152,29 @@ def initial_amplitude(solver):
     return solver.get_psi(1)
 
 
+def born_field(params, src, eps=0.05):
+    """First Born approximation of the scattered field: in the solver's units
+    (lengths scaled by k, so the incident wave number is 1) the steady state
+    obeys (lap + 1) A = (V/E0) A, and A_s = G * ((V/E0) A_init) with the
+    outgoing Green's function, here -1 / (q^2 - 1 - i eps) on a grid
+    zero-padded to twice the size (eps damps the periodic images)."""
+    N = src.shape
+    pad = tuple(2 * n for n in N)
+    S = np.fft.fftn(src, s=pad)
+    q2 = 0.0
+    for a in range(3):
+        q = 2.0 * np.pi * np.fft.fftfreq(pad[a], d=params.d[a])
+        shp = [1, 1, 1]
+        shp[a] = pad[a]
+        q2 = q2 + (q * q).reshape(shp)
+    A = np.fft.ifftn(-S / (q2 - 1.0 - 1j * eps))
+    return A[:N[0], :N[1], :N[2]]
+
+
 def features(params, vbox, a_init):
     """Network input [N][N][N][16]: V/E0, Re and Im of A_init, (V/E0) A_init,
-    k/4, the incidence direction, zeros."""
+    k/4, the incidence direction, Re and Im of the first Born scattered
+    field, zeros."""
     V = potential_grid(params, vbox) / params.E0
     k = float(np.sqrt(2.0 * params.E0))
     th, ph = params.theta, params.phi
@@ -167,6 +187,9 @@ def features(params, vbox, a_init):
     x[..., 4] = V * a_init.imag
     x[..., 5] = k / 4.0
     x[..., 6], x[..., 7], x[..., 8] = khat
+    born = born_field(params, V * a_init)
+    x[..., 9] = born.real
+    x[..., 10] = born.imag
     return x
 
 
@@ -201,38 +224,25 @@ def surface_planes(solver, field):
 
 
 def surface_residual(solver, amp, n0):
-    """Relative mismatch on the virtual surfaces between one Update of the
-    harmonic state A (at n0) and A e^{-i (n0+1) dtau}.  Leaves the solver
-    one step past n0."""
+    """RMS mismatch on the virtual surfaces between one Update of the
+    harmonic state A (at n0) and A e^{-i (n0+1) dtau}: the PSTD residual of
+    A as a time-harmonic solution.  Absolute (the start field vanishes on the
+    surfaces, which lie outside the total-field box).  Leaves the solver one
+    step past n0."""
     set_harmonic_state(solver, amp, n0)
     solver.Update(1)
     want = surface_planes(solver, amp * np.exp(-1j * (n0 + 1) * solver.params.dtau))
     got = surface_planes(solver, solver.get_psi(0))
-    return float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    return float(np.linalg.norm(got - want) / np.sqrt(got.size))
 
 
-def warm_start(solver, net, params, vbox, a_init=None):
-    """Start `solver` (fresh) from the network's prediction, or keep the
-    plain start if the prediction's surface residual is not below that of
-    the start field.  Returns a dict with the choice and both residuals."""
-    if a_init is None:
-        a_init = initial_amplitude(solver)
-    n0 = solver.IterationsToAccumulateSurfaceTerms()
+def predict(net, params, vbox, a_init):
+    """The network's steady-state amplitude A_init + prediction."""
     x = features(params, vbox, a_init)
-    N = x.shape[0]
-    if net.R != N:
-        raise ValueError(f"network resolution {net.R} != grid {N}")
+    if net.R != x.shape[0]:
+        raise ValueError(f"network resolution {net.R} != grid {x.shape[0]}")
     d = net.forward(x[None])[0]
-    amp = a_init + (d[..., 0] + 1j * d[..., 1])
-    r_warm = surface_residual(solver, amp, n0)
-    r_zero = surface_residual(solver, a_init, n0)
-    if r_warm < r_zero:
-        set_harmonic_state(solver, amp, n0)
-        used = "network"
-    else:
-        plain_start(solver, a_init)
-        used = "zero"
-    return {"used": used, "residual_network": r_warm, "residual_zero": r_zero, "n0": n0}
+    return a_init + (d[..., 0] + 1j * d[..., 1])
 
 
 def plain_start(solver, a_init):
@@ -240,3 +250,32 @@ def plain_start(solver, a_init):
     e^{-i dtau} (InitPsi builds the second level that way), time index 1,
     the potential ramp ahead."""
     set_harmonic_state(solver, a_init, 1)
+
+
+def solve_with_warm_start(make_solver, net, params, vbox, tol, max_periods, probe=2):
+    """Run to the steady-state criterion from the network's prediction, with
+    the fallback rule: both starts run `probe` one-period windows, and the
+    one whose last surface change (the PSTD residual of the surface DFT over
+    the first period) is smaller continues.  make_solver() returns a fresh
+    solver of the problem.  Returns the continuing solver and a report whose
+    'periods' counts every window run (both probes included)."""
+    sw = make_solver()
+    a_init = initial_amplitude(sw)
+    n0 = sw.IterationsToAccumulateSurfaceTerms()
+    set_harmonic_state(sw, predict(net, params, vbox, a_init), n0)
+    cw, pw, _, hw = sw.run_to_steady_state(tol=tol, max_periods=probe)
+    if cw:
+        return sw, {"start": "network", "converged": True, "periods": pw, "residual_network": float(hw[-1]),
+                    "residual_plain": None}
+    sz = make_solver()
+    cz, pz, _, hz = sz.run_to_steady_state(tol=tol, max_periods=probe)
+    use_net = hw[-1] < hz[-1]
+    s, other = (sw, sz) if use_net else (sz, sw)
+    other.close()
+    done = pw + pz
+    conv = cz and not use_net
+    if not conv:
+        conv, pc, _, _ = s.run_to_steady_state(tol=tol, max_periods=max(1, max_periods - probe))
+        done += pc
+    return s, {"start": "network" if use_net else "zero", "converged": bool(conv), "periods": done,
+               "residual_network": float(hw[-1]), "residual_plain": float(hz[-1])}
