@@ -160,14 +160,14 @@ def born_field(params, src, eps=0.05):
     zero-padded to twice the size (eps damps the periodic images)."""
     N = src.shape
     pad = tuple(2 * n for n in N)
-    S = np.fft.fftn(src, s=pad)
+    S = np.fft.fftn(src, s=pad, axes=(0, 1, 2))
     q2 = 0.0
     for a in range(3):
         q = 2.0 * np.pi * np.fft.fftfreq(pad[a], d=params.d[a])
         shp = [1, 1, 1]
         shp[a] = pad[a]
         q2 = q2 + (q * q).reshape(shp)
-    A = np.fft.ifftn(-S / (q2 - 1.0 - 1j * eps))
+    A = np.fft.ifftn(-S / (q2 - 1.0 - 1j * eps), axes=(0, 1, 2))
     return A[:N[0], :N[1], :N[2]]
 
 
