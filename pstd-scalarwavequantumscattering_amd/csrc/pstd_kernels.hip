@@ -48,7 +48,18 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
 }
 
 constexpr int PT = 32;      // complex points per x/y tile (two 16-point blocks)
-constexpr int LT = 32;      // lines per z tile (two 16-line blocks)
+constexpr int LT = 32;      // lines per z tile of the LDS variant (two 16-line blocks)
+// Lines per z tile and columns per x/y tile of the register variants: one
+// 16-wide MFMA block (the bodies also take two).  Against 32, a 16-wide tile
+// halves the prefetch and accumulator registers: at L = 80 the FP64 passes
+// go from 2 to 3 (z) and 4 (x, y) waves per SIMD and the FP32 passes to 4
+// and 5, which measured 18-22 % faster per step in both precisions.
+template <typename T> struct LineTile {
+    static constexpr int LT = 16;
+};
+template <typename T> struct PlaneTile {
+    static constexpr int PT = 16;
+};
 constexpr int KBMAX = 24;   // register variant: L <= 96
 // The register variants are instantiated for a compile-time number of 4-row
 // k blocks KBC (16, 20, 24: L = 64, 80, 96), which sizes the operator
@@ -56,7 +67,6 @@ constexpr int KBMAX = 24;   // register variant: L <= 96
 // read at run time).
 template <int KBC> struct KbShape {
     static constexpr int KBM = KBC ? KBC : KBMAX;               // fragments per lane
-    static constexpr int NL = (4 * KBM * PT + 255) / 256;       // tile loads per thread (>= 256 threads)
 };
 
 // Pointers read from a device-side argument table (the batched passes) are
@@ -99,11 +109,12 @@ template <typename T> size_t line_lds_bytes(int MB, int KB) {
     return align16((size_t)MB * KB * 64 * sizeof(T)) + align16((size_t)LT * (4 * KB + 1) * 2 * sizeof(T)) +
            LT * LINEINFO_BYTES;
 }
-template <typename T> size_t plane_reg_lds_bytes(int KB) { return (size_t)4 * KB * PT * 2 * sizeof(T); }
+template <typename T> size_t plane_reg_lds_bytes(int KB) { return (size_t)4 * KB * PlaneTile<T>::PT * 2 * sizeof(T); }
 constexpr size_t ZSRC_BYTES = 48;
 constexpr int ZSRC_ROWS = 128;   // 16-row blocks (<= 8) x 16 core rows
 template <typename T> size_t line_reg_lds_bytes(int KB) {
-    return align16((size_t)LT * (4 * KB + 1) * 2 * sizeof(T)) + LT * LINEINFO_BYTES + ZSRC_ROWS * ZSRC_BYTES;
+    constexpr int LTT = LineTile<T>::LT;
+    return align16((size_t)LTT * (4 * KB + 1) * 2 * sizeof(T)) + LTT * LINEINFO_BYTES + ZSRC_ROWS * ZSRC_BYTES;
 }
 bool reg_variant_ok(int MB, int KB) { return KB <= KBMAX && MB <= 8; }
 
@@ -114,8 +125,8 @@ template <typename C2> __device__ __forceinline__ C2 czero() {
     return v;
 }
 
-// psi0[core row m][col] += -i U for the two 16-point blocks of a x/y tile
-template <typename T, typename P>
+// psi0[core row m][col] += -i U for the NHB 16-point blocks of a x/y tile
+template <typename T, int NHB, typename P>
 __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, int o, long c0,
                                                long core1, int ncore, int wave, int kq, int cq,
                                                const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
@@ -132,7 +143,7 @@ __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, in
             v.y -= ar0[q];
             stv(dst + cq, v);
         }
-        if (c0 + 16 + cq < a.ncols) {
+        if (NHB == 2 && c0 + 16 + cq < a.ncols) {
             C2 v = ldv(dst + 16 + cq);
             v.x += ai1[q];
             v.y -= ar1[q];
@@ -264,7 +275,7 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
     stv(dst, acc);
 }
 
-template <typename T>
+template <typename T, int NHB>
 __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<T> &clk, const typename VT<T>::c2 *Ls,
                                               const LineInfo *LI,
                                               int LS, const ZInfo &zi, const ZSrc &zs, int z, int m, int i1, int kq,
@@ -273,7 +284,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
     typedef typename VT<T>::c2 C2;
     auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
 #pragma unroll
-    for (int hb = 0; hb < 2; ++hb) {
+    for (int hb = 0; hb < NHB; ++hb) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int line = hb * 16 + VT<T>::row(kq, q);
@@ -294,13 +305,16 @@ template <typename T, int KBC>
 __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
-    constexpr int KBM = KbShape<KBC>::KBM, NLD = KbShape<KBC>::NL;
+    constexpr int KBM = KbShape<KBC>::KBM;
+    constexpr int PTT = PlaneTile<T>::PT, NHB = PTT / 16;
+    constexpr int NLD = (4 * KBM * PTT + 255) / 256;   // tile loads per thread (>= 256 threads)
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     C2 *Bs = reinterpret_cast<C2 *>(smem);
     const int KB = KBC ? KBC : a.KB, Kp = 4 * KB;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int ntiles = a.outer_count * a.col_tiles;
+    const int ctiles = (a.ncols + PTT - 1) / PTT;
+    const int ntiles = a.outer_count * ctiles;
     // blocks past a (smaller) problem's last tile leave before loading anything
     if (bx >= ntiles) return;
     const int mbr = a.seg.mb[r];
@@ -311,19 +325,19 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
     for (int kb = 0; kb < KBM; ++kb) af[kb] = (active && (KBC || kb < KB)) ? op[kb * 64] : T(0);
     const long fft1 = a.seg.fft1[r], core1 = a.seg.core1[r];
     const int ncore = a.seg.ncore[r];
-    const int nel = Kp * PT;
+    const int nel = Kp * PTT;
     auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
     auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
     const int kq = lane >> 4, cq = lane & 15;
     C2 pre[NLD];
     auto fetch = [&](int t) {
-        const int o = t / a.col_tiles;
-        const long c0 = (long)(t - o * a.col_tiles) * PT;
+        const int o = t / ctiles;
+        const long c0 = (long)(t - o * ctiles) * PTT;
         auto src = psi + (long)o * a.outer_stride + fft1 * a.row_stride + c0;
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
             const int i = tid + j * nthr;
-            const int l = i / PT, c = i % PT;
+            const int l = i / PTT, c = i % PTT;
             C2 v = czero<C2>();
             if (i < nel && l < a.L && c0 + c < a.ncols) v = ldv(src + (long)l * a.row_stride + c);
             pre[j] = v;
@@ -332,8 +346,8 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
     int t = bx;
     if (t < ntiles) fetch(t);
     for (; t < ntiles; t += gx) {
-        const int o = t / a.col_tiles;
-        const long c0 = (long)(t - o * a.col_tiles) * PT;
+        const int o = t / ctiles;
+        const long c0 = (long)(t - o * ctiles) * PTT;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -344,32 +358,28 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
         if (t + gx < ntiles) fetch(t + gx);
         if (active) {
             ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
-            const C2 *Bw = Bs + kq * PT + cq;
+            const C2 *Bw = Bs + kq * PTT + cq;
 #pragma unroll
             for (int kb = 0; kb < KBM; ++kb) {
                 if (KBC || kb < KB) {
-                    const C2 b0 = Bw[kb * 4 * PT];
-                    const C2 b1 = Bw[kb * 4 * PT + 16];
+                    const C2 b0 = Bw[kb * 4 * PTT];
                     ar0 = mfma16(af[kb], b0.x, ar0);
                     ai0 = mfma16(af[kb], b0.y, ai0);
-                    ar1 = mfma16(af[kb], b1.x, ar1);
-                    ai1 = mfma16(af[kb], b1.y, ai1);
+                    if (NHB == 2) {
+                        const C2 b1 = Bw[kb * 4 * PTT + 16];
+                        ar1 = mfma16(af[kb], b1.x, ar1);
+                        ai1 = mfma16(af[kb], b1.y, ai1);
+                    }
                 }
             }
-            plane_epilogue<T>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
+            plane_epilogue<T, NHB>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
         }
     }
 }
 
-// Waves per SIMD the register variants are compiled for: the FP32 z pass
-// fits three (168 VGPRs) when its fragments are sized at compile time.
-template <typename T, int KBC> constexpr int line_waves() { return (sizeof(T) == 4 && KBC && KBC <= 20) ? 3 : 1; }
-// The FP32 x/y pass at KB = 20 is compiled for four (128 VGPRs, a few
-// spills, measured slightly faster than three); FP64 spills too much to gain.
-template <typename T, int KBC> constexpr int plane_waves() { return (sizeof(T) == 4 && KBC == 20) ? 4 : 1; }
 
 template <typename T, int KBC>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(plane_waves<T, KBC>()))) k_lap_plane_reg(PlaneArgs<T> a) {
+__global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
     plane_reg_body<T, KBC>(a, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
@@ -383,15 +393,18 @@ template <typename T, int KBC>
 __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<T> &clk, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
-    constexpr int KBM = KbShape<KBC>::KBM, NLD = KbShape<KBC>::NL;
+    constexpr int KBM = KbShape<KBC>::KBM;
+    constexpr int LTT = LineTile<T>::LT, NHB = LTT / 16;
+    constexpr int NLD = (LTT * 4 * KBM + 255) / 256;   // tile loads per thread (>= 256 threads)
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int KB = KBC ? KBC : a.KB, Kp = 4 * KB, LS = Kp + 1;
     C2 *Ls = reinterpret_cast<C2 *>(smem);
-    LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)LT * LS * sizeof(C2)));
-    ZSrc *ZT = reinterpret_cast<ZSrc *>(smem + align16((size_t)LT * LS * sizeof(C2)) + LT * LINEINFO_BYTES);
+    LineInfo *LI = reinterpret_cast<LineInfo *>(smem + align16((size_t)LTT * LS * sizeof(C2)));
+    ZSrc *ZT = reinterpret_cast<ZSrc *>(smem + align16((size_t)LTT * LS * sizeof(C2)) + LTT * LINEINFO_BYTES);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    if (bx >= a.line_tiles) return;
+    const int ntiles = (int)((a.nlines + LTT - 1) / LTT);
+    if (bx >= ntiles) return;
     const int mbr = a.seg.mb[r];
     const long fft1 = a.seg.fft1[r];
     const int core1 = a.seg.core1[r], ncore = a.seg.ncore[r];
@@ -417,14 +430,14 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
     T af[KBM];
 #pragma unroll
     for (int kb = 0; kb < KBM; ++kb) af[kb] = (active && (KBC || kb < KB)) ? op[kb * 64] : T(0);
-    const int nel = LT * Kp;
+    const int nel = LTT * Kp;
     auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
     auto rec = glb(reinterpret_cast<const nat_i4 *>(a.linfo));
     C2 pre[NLD];
     nat_i4 lpre = {0, 0, 0, 0};   // 16-byte quarter tid % 4 of line record tid / 4
     auto fetch = [&](int t) {
-        const long q0 = (long)t * LT;
-        if (tid < 4 * LT) {
+        const long q0 = (long)t * LTT;
+        if (tid < 4 * LTT) {
             const long q = q0 + (tid >> 2);
             if (q < a.nlines) lpre = rec[q * 4 + (tid & 3)];
             else lpre = ((tid & 3) == 3) ? nat_i4{-1, -1, -1, 0} : nat_i4{0, 0, 0, 0};   // base = -1
@@ -439,8 +452,8 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
         }
     };
     int t = bx;
-    if (t < a.line_tiles) fetch(t);
-    for (; t < a.line_tiles; t += gx) {
+    if (t < ntiles) fetch(t);
+    for (; t < ntiles; t += gx) {
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -450,9 +463,9 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
                 Ls[line * LS + l] = pre[j];
             }
         }
-        if (tid < 4 * LT) reinterpret_cast<nat_i4 *>(LI)[tid] = lpre;
+        if (tid < 4 * LTT) reinterpret_cast<nat_i4 *>(LI)[tid] = lpre;
         __syncthreads();
-        if (t + gx < a.line_tiles) fetch(t + gx);
+        if (t + gx < ntiles) fetch(t + gx);
         if (!active) continue;
         ACC ar0 = {0, 0, 0, 0}, ai0 = {0, 0, 0, 0}, ar1 = {0, 0, 0, 0}, ai1 = {0, 0, 0, 0};
         const C2 *L0 = Ls + cq * LS + kq;
@@ -461,20 +474,21 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
         for (int kb = 0; kb < KBM; ++kb) {
             if (KBC || kb < KB) {
                 const C2 p0 = L0[4 * kb];
-                const C2 p1 = L1[4 * kb];
                 ar0 = mfma16(p0.x, af[kb], ar0);
                 ai0 = mfma16(p0.y, af[kb], ai0);
-                ar1 = mfma16(p1.x, af[kb], ar1);
-                ai1 = mfma16(p1.y, af[kb], ai1);
+                if (NHB == 2) {
+                    const C2 p1 = L1[4 * kb];
+                    ar1 = mfma16(p1.x, af[kb], ar1);
+                    ai1 = mfma16(p1.y, af[kb], ai1);
+                }
             }
         }
-        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T, NHB>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
 }
 
 template <typename T, int KBC>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(line_waves<T, KBC>())))
-k_lap_line_reg(LineArgs<T> a) {
+__global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
     const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
     line_reg_body<T, KBC>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
 }
@@ -520,8 +534,7 @@ __global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restr
 }
 
 template <typename T, int KBC>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(line_waves<T, KBC>())))
-k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
+__global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
                                                     const double *__restrict__ clock, const int *__restrict__ step,
                                                     int nprob) {
     const int2 br = map[blockIdx.y];
@@ -625,7 +638,7 @@ __global__ void __launch_bounds__(1024) k_lap_plane_lds(PlaneArgs<T> a) {
                 ar1 = mfma16(af, b1.x, ar1);
                 ai1 = mfma16(af, b1.y, ai1);
             }
-            plane_epilogue<T>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
+            plane_epilogue<T, 2>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
         }
     }
 }
@@ -682,7 +695,7 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             ar1 = mfma16(p1.x, af, ar1);
             ai1 = mfma16(p1.y, af, ai1);
         }
-        if (lane_ok) line_epilogue<T>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T, 2>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1);
     }
 }
 
