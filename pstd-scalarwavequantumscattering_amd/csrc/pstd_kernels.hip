@@ -245,20 +245,19 @@ __device__ __forceinline__ ZSrc z_src(const LineArgs<T> &a, int z) {
 // (:1306-1419, regenerated from separable FP64 tables and the FP64 time
 // phase) and the ABC mask (:1421-1435).
 // One output point of the fused z pass: psi0 at dst, U = alpha D2 (W psi)
-// at the point (alpha folded), pn = psi^n at the point.
+// at the point (alpha folded), pn = psi^n at the point, v = V~ at the point
+// (0 outside the potential box; the ramp factor is applied here).
 template <typename T, typename P>
 __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &clk, const LineInfo &li,
                                         const ZInfo &zi, const ZSrc &zs, P dst, T ur, T ui,
-                                        typename VT<T>::c2 pn) {
+                                        typename VT<T>::c2 pn, T v) {
     typedef typename VT<T>::c2 C2;
     C2 acc = ldv(dst);
     acc.x += ui;
     acc.y -= ur;
-    if ((zi.flags & 1) && li.vrow >= 0) {
-        const T v = clk.vfac * glb(a.vbox)[li.vrow + zi.zoff];
-        acc.x += v * pn.y;
-        acc.y -= v * pn.x;
-    }
+    v *= clk.vfac;
+    acc.x += v * pn.y;
+    acc.y -= v * pn.x;
     if ((zi.flags & 2) && (li.flags & 1) && !((zi.flags & 4) && (li.flags & 2))) {
         const double2 az = zs.az, ez = zs.ez;
         const double pz = zs.pz;
@@ -275,12 +274,21 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
     stv(dst, acc);
 }
 
-template <typename T, int NHB>
+// V~ at the point of line li and z (a global load; the register variant
+// prefetches it instead)
+template <typename T>
+__device__ __forceinline__ T z_potential(const LineArgs<T> &a, const LineInfo &li, const ZInfo &zi) {
+    return ((zi.flags & 1) && li.vrow >= 0) ? glb(a.vbox)[li.vrow + zi.zoff] : T(0);
+}
+
+// PV: vq[q] holds V~ of result element q (NHB = 1), prefetched by the caller
+template <typename T, int NHB, bool PV>
 __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<T> &clk, const typename VT<T>::c2 *Ls,
                                               const LineInfo *LI,
                                               int LS, const ZInfo &zi, const ZSrc &zs, int z, int m, int i1, int kq,
                                               const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
-                                              const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1) {
+                                              const typename VT<T>::acc &ar1, const typename VT<T>::acc &ai1,
+                                              const T (&vq)[4]) {
     typedef typename VT<T>::c2 C2;
     auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
 #pragma unroll
@@ -290,8 +298,9 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
             const int line = hb * 16 + VT<T>::row(kq, q);
             const LineInfo &li = LI[line];
             if (li.base < 0) continue;
+            const T v = PV ? vq[q] : z_potential<T>(a, li, zi);
             z_point<T>(a, clk, li, zi, zs, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
-                       Ls[line * LS + i1 + m]);
+                       Ls[line * LS + i1 + m], v);
         }
     }
 }
@@ -451,8 +460,37 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
             pre[j] = v;
         }
     };
+    // V~ of this lane's four result elements (lines row(kq, q) of a tile, at
+    // z) for the next tile, loaded after each epilogue; the box row follows
+    // from the line index (x Ny + y)
+    static_assert(NHB == 1, "potential prefetch covers one 16-line block");
+    T vq[4] = {T(0), T(0), T(0), T(0)};
+    const bool vz = lane_ok && (zi.flags & 1);
+    const int nyv = a.vb2[1] - a.vb1[1] + 1, nzv = a.vb2[2] - a.vb1[2] + 1;
+    auto vbox = glb(a.vbox);
+    auto fetch_v = [&](int t) {
+        // (x, y) of the tile's first line by one division; rows step from it
+        const int q0 = t * LTT;
+        const int x0 = q0 / a.Ny, y0 = q0 - x0 * a.Ny;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = VT<T>::row(kq, q);
+            int x = x0, y = y0 + r;
+            while (y >= a.Ny) {   // r < 16: at most ceil(16 / Ny) passes
+                y -= a.Ny;
+                ++x;
+            }
+            T v = T(0);
+            if (vz && (long)q0 + r < a.nlines && x >= a.vb1[0] && x <= a.vb2[0] && y >= a.vb1[1] && y <= a.vb2[1])
+                v = vbox[((long)(x - a.vb1[0]) * nyv + (y - a.vb1[1])) * nzv + zi.zoff];
+            vq[q] = v;
+        }
+    };
     int t = bx;
-    if (t < ntiles) fetch(t);
+    if (t < ntiles) {
+        fetch(t);
+        fetch_v(t);
+    }
     for (; t < ntiles; t += gx) {
         __syncthreads();
 #pragma unroll
@@ -483,7 +521,8 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
                 }
             }
         }
-        if (lane_ok) line_epilogue<T, NHB>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1);
+        if (lane_ok) line_epilogue<T, NHB, true>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1, vq);
+        if (t + gx < ntiles) fetch_v(t + gx);
     }
 }
 
@@ -695,7 +734,8 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             ar1 = mfma16(p1.x, af, ar1);
             ai1 = mfma16(p1.y, af, ai1);
         }
-        if (lane_ok) line_epilogue<T, 2>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1);
+        const T vnone[4] = {T(0), T(0), T(0), T(0)};
+        if (lane_ok) line_epilogue<T, 2, false>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1, vnone);
     }
 }
 
@@ -983,7 +1023,8 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
             const int z = core1 + m;
             const ZInfo zi = z_info<T>(a, z);
             const C2 u = S[(i1 + m) * LDW + c];
-            z_point<T>(a, clk, li, zi, z_src<T>(a, z), psi0 + li.base + z, u.x, u.y, psi[li.base + z]);
+            z_point<T>(a, clk, li, zi, z_src<T>(a, z), psi0 + li.base + z, u.x, u.y, psi[li.base + z],
+                       z_potential<T>(a, li, zi));
         }
     }
 }
