@@ -184,7 +184,7 @@ __device__ __forceinline__ void line_info(const LineArgs<T> &a, const Clock<T> &
     }
     const int x = (int)qg / a.Ny, y = (int)qg - x * a.Ny;
     li.base = qg * a.Nz;
-    li.gxy = (double)glb(a.g[0])[x] * (double)glb(a.g[1])[y];
+    li.gxy = glb(a.g[0])[x] * glb(a.g[1])[y];
     li.vrow = -1;
     if (a.use_v && x >= a.vb1[0] && x <= a.vb2[0] && y >= a.vb1[1] && y <= a.vb2[1])
         li.vrow = ((x - a.vb1[0]) * (a.vb2[1] - a.vb1[1] + 1) + (y - a.vb1[1])) * (a.vb2[2] - a.vb1[2] + 1);
@@ -221,7 +221,7 @@ static_assert(sizeof(ZSrc) == ZSRC_BYTES, "ZSrc must fit its LDS slot");
 template <typename T>
 __device__ __forceinline__ ZInfo z_info(const LineArgs<T> &a, int z) {
     ZInfo zi;
-    zi.gz = (double)glb(a.g[2])[z];
+    zi.gz = glb(a.g[2])[z];
     zi.zoff = z - a.vb1[2];
     const bool inv = a.use_v && z >= a.vb1[2] && z <= a.vb2[2];
     const bool inbox = z >= a.k1a[2] && z <= a.k2b[2];
@@ -268,9 +268,10 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
         acc.x -= (T)(er * ur2 - ei * ui2);
         acc.y -= (T)(er * ui2 + ei * ur2);
     }
-    const T gg = (T)(li.gxy * zi.gz);
-    acc.x *= gg;
-    acc.y *= gg;
+    // ABC in FP64 (one rounding of the product in FP32)
+    const double gg = li.gxy * zi.gz;
+    acc.x = (T)(acc.x * gg);
+    acc.y = (T)(acc.y * gg);
     stv(dst, acc);
 }
 

@@ -75,8 +75,8 @@ struct LineArgs {
     // vanishes strictly inside (ksi1b, ksi2a) on all three axes
     int k1a[3], k1b[3], k2a[3], k2b[3];
     double sin_t, cos_t;
-    // ABC
-    const T *g[3];
+    // ABC (FP64 in both precisions)
+    const double *g[3];
     // time-independent per-line terms (64-byte records, nlines of them) built
     // once by launch_line_table; read by the register variant
     const void *linfo;

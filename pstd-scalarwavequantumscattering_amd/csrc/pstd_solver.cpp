@@ -212,7 +212,7 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
             T.opoff[r] = (int)s->lay[a].off[r];
         }
         std::vector<double> gv = abc_vector(g, a, p->dtau, p->cap_U0, p->cap_alpha);
-        if ((rc = upload_real(s, s->gvec[a], gv.data(), gv.size(), "abc"))) { delete s; return rc; }
+        if ((rc = s->upload(s->gvec[a], gv.data(), gv.size() * 8, "abc"))) { delete s; return rc; }   // FP64 always
         TfsfAxis tf = tfsf_axis(g, a, p->dtau, p->n_ksi);
         if ((rc = s->upload(s->tfP[a], tf.P.data(), tf.P.size() * 8, "tfsf"))) { delete s; return rc; }
         if ((rc = s->upload(s->tfA[a], tf.A.data(), tf.A.size() * 16, "tfsf"))) { delete s; return rc; }
@@ -361,7 +361,7 @@ LineArgs<T> line_args(pstd_solver *s, const T *psi, T *psi0, long ntau) {
         a.tfP[d] = s->tfP[d]; a.tfA[d] = s->tfA[d]; a.tfE[d] = s->tfE[d];
         a.k1a[d] = (int)std::min(g.ksi1a[d], g.ksi2a[d]); a.k1b[d] = (int)g.ksi1b[d];
         a.k2a[d] = (int)g.ksi2a[d]; a.k2b[d] = (int)std::max(g.ksi1b[d], g.ksi2b[d]);
-        a.g[d] = reinterpret_cast<const T *>(s->gvec[d]);
+        a.g[d] = reinterpret_cast<const double *>(s->gvec[d]);
     }
     a.linfo = s->linfo;
     step_clock(s, ntau, a.vfac, a.sin_t, a.cos_t);
