@@ -744,26 +744,44 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
 // Surface capture (AccumulateSurfaceTerms, PSTD_QScat.cpp:1738-2052): per
 // surface point, the owning segment's windowed line dotted with the PS and
 // FD8 weight vectors, psi at the surface, all times e^{i tau}, accumulated
-// in FP64.
+// in FP64.  SURF_LANES lanes share a point: lane j sums the taps l = j mod
+// SURF_LANES and an xor-shuffle reduction inside the aligned lane group
+// combines them, so a launch has SURF_LANES times the waves to hide the
+// line reads' latency.
 // ---------------------------------------------------------------------------
+constexpr int SURF_LANES = 8;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_surface(const T *psi_, SurfArgs a) {
     typedef typename VT<T>::c2 C2;
     const SurfFace &F = a.f[blockIdx.y];
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = gt / SURF_LANES;
+    const int j = (int)(gt % SURF_LANES);
     const long npts = (long)F.nb * F.nc;
-    if (p >= npts) return;
-    const long u = p / F.nc, v = p - u * F.nc;
+    const bool live = p < npts;   // every lane takes part in the shuffles
+    const long pp = live ? p : 0;
+    const long u = pp / F.nc, v = pp - u * F.nc;
     const C2 *base = reinterpret_cast<const C2 *>(psi_) + u * F.stride_b + v * F.stride_c + F.fft1 * F.stride_l;
     double pr = 0.0, pi = 0.0, fr = 0.0, fi = 0.0;
-    for (int l = 0; l < a.L; ++l) {
-        const C2 c = base[l * F.stride_l];
-        const double w = F.ps[l], w2 = F.fd[l];
-        pr += w * (double)c.x;
-        pi += w * (double)c.y;
-        fr += w2 * (double)c.x;
-        fi += w2 * (double)c.y;
+    if (live) {
+        for (int l = j; l < a.L; l += SURF_LANES) {
+            const C2 c = base[l * F.stride_l];
+            const double w = F.ps[l], w2 = F.fd[l];
+            pr += w * (double)c.x;
+            pi += w * (double)c.y;
+            fr += w2 * (double)c.x;
+            fi += w2 * (double)c.y;
+        }
     }
+#pragma unroll
+    for (int o = 1; o < SURF_LANES; o <<= 1) {
+        pr += __shfl_xor(pr, o);
+        pi += __shfl_xor(pi, o);
+        fr += __shfl_xor(fr, o);
+        fi += __shfl_xor(fi, o);
+    }
+    if (!live || j != 0) return;
     const C2 c = base[(long)F.li * F.stride_l];
     const double cr = (double)c.x, ci = (double)c.y;
     double2 *acc = F.acc;
@@ -1110,7 +1128,7 @@ hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s) {
 }
 template <typename T>
 hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStream_t s) {
-    dim3 grid((unsigned)((max_pts + 255) / 256), 2), block(256);
+    dim3 grid((unsigned)((max_pts * SURF_LANES + 255) / 256), 2), block(256);
     hipLaunchKernelGGL(k_surface<T>, grid, block, 0, s, psi, a);
     return hipGetLastError();
 }
