@@ -194,6 +194,30 @@ int pstd_reset_surface_terms(pstd_solver *s);
 int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long *periods_out, double *history,
                              int *converged_out);
 
+/* ---- diagnostics ------------------------------------------------------ */
+
+/* CPSTD_QScat::Save_g (PSTD_QScat.cpp:2191-2253): for every subdomain (rank
+ * in MPI's row-major Cartesian order, z fastest), text files
+ * Rank<r>_g1.dat, _g2.dat, _g3.dat in directory dir (NULL or "": current),
+ * lines "i<TAB>g" with i the core-local index and g as %.16e.  The
+ * reference loops to the FFT length over a core-length vector (:2205, an
+ * out-of-bounds read); here the core length is written. */
+int pstd_save_g(const pstd_solver *s, const char *dir);
+
+/* CPSTD_QScat::SavePsi_{X,Y,Z}Plane(prefix, long index)
+ * (PSTD_QScat.cpp:2260-2388) for axis 0, 1, 2: for every subdomain whose
+ * core holds that plane, <prefix><nTau>_Rank<r>_r<axis><index>.dat and
+ * _i<axis><index>.dat with the real and imaginary parts of the newest psi
+ * over the subdomain's local box on the plane as the exchange
+ * (PSTD_QScat.cpp:1539-1731) leaves it: face halos windowed, edges and
+ * corners zero.  One row per local index of the first in-plane axis,
+ * tab-terminated %.16e values.  A plane outside every core writes nothing.
+ * A file that cannot be opened is reported on stderr as "<name> failed" and
+ * skipped, as in the reference; the call then returns PSTD_EIO (so does
+ * pstd_save_g).  The coordinate overloads (index = (long)(x0 / dx +
+ * (N - 1) / 2), :2255-2258) are the caller's. */
+int pstd_save_psi_plane(pstd_solver *s, int axis, long index, const char *prefix);
+
 /* ---- state access (tests, warm start, diagnostics) ------------------ */
 
 /* Copy psi^n (level 0) or psi^(n-1) (level 1) of the whole global grid to

@@ -593,6 +593,97 @@ int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix) {
     return pstd_reset_surface_terms(s);
 }
 
+// Subdomain rank in MPI's row-major Cartesian order (qscat.cpp:156-157,
+// PSTD_QScat.cpp:253-260): z fastest.
+static long rank_of(const Geometry &g, int xr, int yr, int zr) {
+    return ((long)xr * g.ax[1].P + yr) * g.ax[2].P + zr;
+}
+
+int pstd_save_g(const pstd_solver *s, const char *dir) {
+    if (!s) return fail(PSTD_EINVAL, "null solver");
+    const Geometry &g = s->g;
+    const std::string d = (dir && *dir) ? std::string(dir) + "/" : std::string();
+    std::vector<double> gv[3];
+    for (int a = 0; a < 3; ++a) gv[a] = abc_vector(g, a, s->p.dtau, s->p.cap_U0, s->p.cap_alpha);
+    std::string bad;
+    for (int xr = 0; xr < g.ax[0].P; ++xr)
+        for (int yr = 0; yr < g.ax[1].P; ++yr)
+            for (int zr = 0; zr < g.ax[2].P; ++zr) {
+                const int rr[3] = {xr, yr, zr};
+                const long rank = rank_of(g, xr, yr, zr);
+                for (int a = 0; a < 3; ++a) {
+                    const std::string name = d + "Rank" + std::to_string(rank) + "_g" + std::to_string(a + 1) + ".dat";
+                    FILE *fp = std::fopen(name.c_str(), "w");
+                    if (!fp) {   // reported and skipped, as the reference does
+                        std::fprintf(stderr, "%s failed\n", name.c_str());
+                        if (bad.empty()) bad = name;
+                        continue;
+                    }
+                    // the core-length vector (the reference loops to the FFT length, reading past it)
+                    const long c1 = g.ax[a].core1[rr[a]], c2 = g.ax[a].core2[rr[a]];
+                    for (long i = c1; i <= c2; ++i) std::fprintf(fp, "%ld\t%.16e\n", i - c1, gv[a][i]);
+                    if (std::fclose(fp) != 0 && bad.empty()) bad = name;
+                }
+            }
+    return bad.empty() ? 0 : fail(PSTD_EIO, bad + " failed");
+}
+
+int pstd_save_psi_plane(pstd_solver *s, int axis, long index, const char *prefix) {
+    if (!s || !prefix || axis < 0 || axis > 2) return fail(PSTD_EINVAL, "bad plane arguments");
+    const Geometry &g = s->g;
+    if (index < 0 || index >= g.ax[axis].N) return 0;   // outside every core: nothing written
+    std::vector<double> psi(s->npts() * 2);
+    int rc = pstd_get_psi(s, 0, psi.data(), psi.size());
+    if (rc) return rc;
+    const long Ny = g.ax[1].N, Nz = g.ax[2].N;
+    const int b = (axis == 0) ? 1 : 0, c = (axis == 2) ? 1 : 2;   // in-plane axes, row and column
+    const char tag = "xyz"[axis];
+    std::string bad;
+    for (int xr = 0; xr < g.ax[0].P; ++xr)
+        for (int yr = 0; yr < g.ax[1].P; ++yr)
+            for (int zr = 0; zr < g.ax[2].P; ++zr) {
+                const int rr[3] = {xr, yr, zr};
+                if (index < g.ax[axis].core1[rr[axis]] || index > g.ax[axis].core2[rr[axis]]) continue;
+                // The local box after the exchange (PSTD_QScat.cpp:1539-1731): the
+                // exchange sends h-1 planes over the core range of the other two
+                // axes, so a face halo holds W(l) times the neighbour's core --
+                // the global field -- and an edge or corner, in the halos of two
+                // axes at once, is never written and stays zero.
+                const std::vector<long double> wb = segment_window(g.ax[b], rr[b]);
+                const std::vector<long double> wc = segment_window(g.ax[c], rr[c]);
+                const long fb = g.ax[b].fft1[rr[b]], fc = g.ax[c].fft1[rr[c]];
+                const long cb1 = g.ax[b].core1[rr[b]] - fb, cb2 = g.ax[b].core2[rr[b]] - fb;
+                const long cc1 = g.ax[c].core1[rr[c]] - fc, cc2 = g.ax[c].core2[rr[c]] - fc;
+                const long rank = rank_of(g, xr, yr, zr);
+                for (int part = 0; part < 2; ++part) {
+                    const std::string name = std::string(prefix) + std::to_string(s->nTau) + "_Rank" +
+                                             std::to_string(rank) + "_" + (part ? "i" : "r") + tag +
+                                             std::to_string(index) + ".dat";
+                    FILE *fp = std::fopen(name.c_str(), "w");
+                    if (!fp) {   // reported and skipped, as the reference does
+                        std::fprintf(stderr, "%s failed\n", name.c_str());
+                        if (bad.empty()) bad = name;
+                        continue;
+                    }
+                    for (long lb = 0; lb < g.ax[b].L; ++lb) {
+                        for (long lc = 0; lc < g.ax[c].L; ++lc) {
+                            long ijk[3];
+                            ijk[axis] = index;
+                            ijk[b] = fb + lb;
+                            ijk[c] = fc + lc;
+                            const long o = ((ijk[0] * Ny + ijk[1]) * Nz + ijk[2]) * 2 + part;
+                            const bool core_b = lb >= cb1 && lb <= cb2, core_c = lc >= cc1 && lc <= cc2;
+                            const double v = (core_b || core_c) ? (double)(wb[lb] * wc[lc]) * psi[o] : 0.0;
+                            std::fprintf(fp, "%.16e\t", v);
+                        }
+                        std::fputc('\n', fp);
+                    }
+                    if (std::fclose(fp) != 0 && bad.empty()) bad = name;
+                }
+            }
+    return bad.empty() ? 0 : fail(PSTD_EIO, bad + " failed");
+}
+
 int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long *periods_out, double *history,
                              int *converged_out) {
     if (!s || !periods_out || !converged_out || max_periods < 1 || !(tol > 0.0))

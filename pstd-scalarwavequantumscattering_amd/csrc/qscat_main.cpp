@@ -2,12 +2,16 @@
 // MI355X library.  One process, one GPU; every subdomain of the input's
 // process grid runs on the device as a segment of one global grid.
 //
-//   qscat_amd -i FILE [--fp32]
+//   qscat_amd -i FILE [--fp32] [--no-planes]
 //
 // The time loop, the one-period accumulation windows (the last period, and
 // the periods ending at steps 20000/40000/60000) and the surface files are
-// those of the reference (qscat.cpp:165-198).  The diagnostic plane dumps
-// (qscat.cpp:160-163, :175-179) are not produced.
+// those of the reference (qscat.cpp:165-198).  So are the diagnostics: the
+// ABC vectors and the x, y, z planes through the origin at start-up
+// (Rank<r>_g<a>.dat, Init<nTau>_Rank<r>_{r,i}{x,y,z}<i>.dat; :160-163) and
+// the planes after every tenth of the run (Iter...; :175-179, skipped when
+// Iter < 10, where the reference divides by zero).  --no-planes leaves the
+// plane files out.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -18,12 +22,12 @@
 extern "C" double pstd_default_potential(double x, double y, double z);
 
 static void show_usage(const char *app) {
-    std::printf("Usage: %s -i FILE [--fp32]\nwhere FILE: input file for model parameters\n", app);
+    std::printf("Usage: %s -i FILE [--fp32] [--no-planes]\nwhere FILE: input file for model parameters\n", app);
 }
 
 int main(int argc, char **argv) {
     std::string openname;
-    bool fp32 = false;
+    bool fp32 = false, planes = true;
     for (int i = 1; i < argc; ++i) {
         std::string arg = argv[i];
         if (arg == "-h" || arg == "--help") {
@@ -31,6 +35,7 @@ int main(int argc, char **argv) {
             return 0;
         }
         if (arg == "--fp32") fp32 = true;
+        if (arg == "--no-planes") planes = false;
         if (arg == "-i") {
             if (i < argc - 1) openname = argv[i + 1];
             if (i == argc - 1 || openname[0] == '-') {
@@ -60,7 +65,25 @@ int main(int argc, char **argv) {
     }
     std::printf("Process topology (x, y, z): %d, %d, %d (segments of one MI355X grid)\n", p.nprocs[0], p.nprocs[1],
                 p.nprocs[2]);
+    pstd_geometry geo;
+    rc = pstd_compute_geometry(&p, &geo);
+    // SavePsi_?Plane(prefix, 0.0): index (long)(0.0 / d + orig), PSTD_QScat.cpp:2255-2258
+    long origin[3];
+    for (int a = 0; a < 3; ++a) origin[a] = (long)(0.0 / p.d[a] + geo.orig[a]);
+    auto save_planes = [&](const char *prefix) {
+        int r = 0;
+        for (int a = 0; a < 3 && !r; ++a) r = pstd_save_psi_plane(s, a, origin[a], prefix);
+        return r;
+    };
+    if (!rc) rc = pstd_save_g(s, "");
+    if (!rc && planes) rc = save_planes("Init");
+    if (rc) {
+        std::fprintf(stderr, "Error %d: %s\n", rc, pstd_last_error());
+        pstd_destroy(s);
+        return 1;
+    }
     const long Iter = p.iterations;
+    const long tenth = Iter / 10;
     const long period = pstd_steps_per_period(s);
     const long every = (Iter >> 7) > 0 ? (Iter >> 7) : 1;
     auto t0 = std::chrono::steady_clock::now();
@@ -75,6 +98,10 @@ int main(int argc, char **argv) {
         }
         rc = pstd_update(s, 1);
         if (rc) break;
+        if (planes && tenth > 0 && i % tenth == 0) {
+            rc = save_planes("Iter");
+            if (rc) break;
+        }
         if (!accum) {
             if (i + period - 1 >= Iter) {
                 iter_th = Iter;

@@ -26,6 +26,7 @@ EXPORTS = [
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
+    "pstd_save_g", "pstd_save_psi_plane",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
 ]
@@ -88,6 +89,10 @@ def library():
         L.pstd_get_method.argtypes = [vp, i]
         L.pstd_autotune.restype = i
         L.pstd_autotune.argtypes = [vp, C.c_char_p, i, dp]
+        L.pstd_save_g.restype = i
+        L.pstd_save_g.argtypes = [vp, C.c_char_p]
+        L.pstd_save_psi_plane.restype = i
+        L.pstd_save_psi_plane.argtypes = [vp, i, l, C.c_char_p]
         L.pstd_run_to_steady_state.restype = i
         L.pstd_run_to_steady_state.argtypes = [vp, C.c_double, l, C.POINTER(l), dp, C.POINTER(i)]
         L.pstd_batch_create.restype = i
@@ -237,6 +242,19 @@ class PSTDQScat:
 
     def reset_surface_terms(self):
         _check(library().pstd_reset_surface_terms(self.h))
+
+    def Save_g(self, directory=""):
+        """PSTD_QScat.cpp:2191-2253 (core-length vectors)."""
+        _check(library().pstd_save_g(self.h, str(directory).encode()))
+
+    def SavePsi_Plane(self, axis, index, prefix):
+        """SavePsi_{X,Y,Z}Plane(prefix, long index), PSTD_QScat.cpp:2260-2388."""
+        _check(library().pstd_save_psi_plane(self.h, int(axis), int(index), str(prefix).encode()))
+
+    def plane_index(self, axis, x0):
+        """The coordinate overloads' index (long)(x0 / d + (N - 1) / 2),
+        PSTD_QScat.cpp:2255-2258 with the origin of :138-140."""
+        return int(x0 / self.params.d[axis] + (self.N[axis] - 1) / 2.0)
 
     def run_to_steady_state(self, tol=1e-6, max_periods=100):
         """Back-to-back one-period surface windows until the relative change
