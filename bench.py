@@ -33,6 +33,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 PKG_DIR = ROOT / "pstd-scalarwavequantumscattering_amd"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "grid-point·steps/s and time-to-steady-state at 1e-6 surface error vs FP64"
 
 
@@ -146,6 +147,35 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
+def far_field_run(pkg, W, n_theta=181, n_phi=360, reps=5):
+    """Far-field transform of the grf224 FP64 surfaces (one-period window
+    after a 20-step start) onto a Gauss-Legendre x uniform sphere grid.
+    The kernel is FP64-FMA bound: 6 FMAs (12 flops) per surface point and
+    direction; the call also clips and weights the faces and copies the
+    directions in and f out."""
+    from pstd_amd import farfield as FF
+    p, V = W.config_grf224(precision=pkg.PSTD_FP64)
+    s = pkg.PSTDQScat(p, V)
+    s.Update(20)
+    s.update_accumulate(s.IterationsToAccumulateSurfaceTerms())
+    dirs, _, _, _ = FF.sphere_quadrature(n_theta, n_phi, axis=FF.incident_direction(p))
+    FF.far_field(s, dirs)                  # warm-up
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        FF.far_field(s, dirs)
+    dt = (time.perf_counter() - t0) / reps
+    g = s.geometry
+    n = [g.vrtl2[a] - g.vrtl1[a] + 1 for a in range(3)]
+    pts = 2 * (n[1] * n[2] + n[0] * n[2] + n[0] * n[1])
+    s.close()
+    flops = 12.0 * pts * len(dirs)
+    return {"workload": f"grf224 FP64 surfaces after one period, {len(dirs)} directions "
+                        f"({n_theta} x {n_phi} sphere grid)", "surface_points": pts, "directions": len(dirs),
+            "ms_per_call": dt * 1e3, "call_tflops_fp64": flops / dt / 1e12,
+            "call_frac_of_fp64_peak": flops / dt / 1e12 / FP64_PEAK_TFLOPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +186,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--steady-periods", type=int, default=60)
     ap.add_argument("--no-steady", action="store_true")
+    ap.add_argument("--no-far-field", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -208,6 +239,8 @@ def main():
         line["fp32_fp64phase"] = extra
     if not args.no_fp32:
         line["batch48_fp32"] = batch_run(pkg, W, 100)
+    if not args.no_far_field:
+        line["far_field"] = far_field_run(pkg, W)
     if not args.no_steady:
         line["time_to_steady_state"] = steady_state(pkg, W, args.steady_periods, 4000)
     if not args.no_cpu:

@@ -1,0 +1,62 @@
+/* Far-field post-processor on the virtual-surface data (GPU, FP64).
+ *
+ * No counterpart in the reference: the reference stops at the surface files
+ * and names their purpose -- the frequency-domain near-to-far-field
+ * transformation of Taflove and Hagness, Computational Electrodynamics, 3rd
+ * ed., p. 335 -- in PSTD_QScat.cpp:1740-1747.  These entry points compute it
+ * for the scalar field, in the solver's scaled units (lengths in lambdabar,
+ * k = 1):
+ *
+ *   A(r) -> f(rhat) e^{i r} / r,
+ *   f(rhat) = -1/(4 pi) oint_S e^{-i rhat.r'} [ i (rhat.n') A + dA/dn' ] dS'
+ *
+ * over the closed box X1V..X2V x Y1V..Y2V x Z1V..Z2V of the surface data, A
+ * the steady-state amplitude of psi = A e^{-i tau} (the surface DFT divided
+ * by its step count) and n' the outward normal.  f is the amplitude for a
+ * unit incident wave e^{i khat.r} with its phase origin at the grid centre,
+ * the origin of the coordinates r = (i - orig) d; the solver's incident wave
+ * has its phase origin at the injection corner instead (PSTD_QScat.cpp:
+ * 624-653), so its scattered field is divided by the incident wave's
+ * amplitude at the centre, pstd_incident_amplitude.  Each face is integrated
+ * with the extended Simpson product rule (Numerical Recipes eq. 4.1.14;
+ * trapezoid on edges shorter than 8 points).  The differential cross
+ * section is |f|^2 in lambdabar^2; the optical theorem reads
+ * sigma_tot = 4 pi Im f(khat).
+ *
+ * dirs holds ndir unit vectors (x, y, z; |n| within 1e-6 of 1, else 22);
+ * f_out receives ndir complex values (re, im).  Returns 0 or a pstd error
+ * code (pstd_last_error). */
+#ifndef PSTD_FARFIELD_H
+#define PSTD_FARFIELD_H
+
+#include "pstd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The solver's incident wave at the grid centre, e^{-i khat.r_inj} with
+ * r_inj = (inj - orig) d the injection corner (PSTD_QScat.cpp:624-653,
+ * :959-1118), as (re, im).  Host only; 22 on an inconsistent geometry. */
+int pstd_incident_amplitude(const pstd_params *p, double out[2]);
+
+/* From the solver's current accumulation window (pstd_accumulate_surface_terms
+ * since the last save or reset), with the derivative of kind deriv
+ * (PSTD_DERIV_PS or PSTD_DERIV_FD), divided by the solver's own incident
+ * amplitude.  22 if no step has been accumulated. */
+int pstd_far_field(pstd_solver *s, int deriv, long ndir, const double *dirs, double *f_out);
+
+/* From a virtual-surface file written by pstd_save_virtual_surfaces or by
+ * the reference (PSTD_QScat.cpp:2390-2555; precision 4 or 8, either
+ * derivative kind), divided by inc (re, im), the incident wave's amplitude
+ * at the grid centre -- pstd_incident_amplitude of the run's parameters;
+ * NULL means 1 (the transform of the data as stored).  The file does not
+ * record the incidence, hence the argument.  3 if the file cannot be read or
+ * its size does not match its header. */
+int pstd_far_field_file(const char *path, const double inc[2], long ndir, const double *dirs, double *f_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

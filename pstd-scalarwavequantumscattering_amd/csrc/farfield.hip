@@ -1,0 +1,433 @@
+// Far-field transformation of the virtual-surface data (the frequency-domain
+// near-to-far-field step that PSTD_QScat.cpp:1740-1747 names as the purpose
+// of the surface DFT; Taflove and Hagness, Computational Electrodynamics,
+// 3rd ed., p. 335), for the scalar field in the solver's scaled units (k = 1):
+//
+//   f(rhat) = -1/(4 pi) oint_S e^{-i rhat.r'} [ i (rhat.n') A + dA/dn' ] dS'
+//
+// over the closed box X1V..X2V x Y1V..Y2V x Z1V..Z2V, n' the outward normal
+// (-e_a on the low face of axis a, +e_a on the high one; the stored
+// derivative is along +axis on both, PSTD_QScat.cpp:1805-1852).  Each face
+// is a product rule over its index rectangle with the extended Simpson
+// weights 17/48 59/48 43/48 49/48 1 ... (Numerical Recipes eq. 4.1.14) on
+// edges of 8 or more points, trapezoid otherwise.
+//
+// On a face with in-plane axes b (rows) and c (columns), every factor of the
+// phase is a power of a per-direction unit complex number:
+//   e^{-i rhat.r'} = e^{-i (r_a x_a + r_b y_0 + r_c z_0)} u^j z^k,
+//   u = e^{-i r_b d_b}, z = e^{-i r_c d_c},
+// so a row is a polynomial in z and the rows a polynomial in u, both
+// evaluated by Horner's rule: 6 FP64 FMAs per surface point and direction,
+// no transcendental in the inner loop.  One lane per direction; the surface
+// data is the same for all lanes of a wave and is read by uniform loads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pstd.h"
+#include "../../include/pstd_farfield.h"
+#include "farfield.hpp"
+
+namespace pstd {
+int report_error(int code, const std::string &m);
+int report_hip_error(hipError_t e, const char *what);
+}  // namespace pstd
+
+#define FFCHK(call, what)                                                 \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return pstd::report_hip_error(e_, what);    \
+    } while (0)
+
+namespace {
+
+constexpr double FOUR_PI = 12.566370614359172953850573533118011536788677597500;
+constexpr int FF_WAVE = 64;
+constexpr int FF_BLOCK_WAVES = 4;    // waves per block, splitting a run of rows
+constexpr int FF_TARGET_WAVES = 32768;   // enough to keep 1024 SIMDs busy through the tail
+constexpr int FF_GROUP = 8;          // surface points loaded per wait
+#ifndef FF_DIRS_PER_LANE
+#define FF_DIRS_PER_LANE 4           // directions per lane (register blocking)
+#endif
+
+// A run of rows of one face: offsets into the compact weighted arrays and the
+// geometry of its first row.
+struct FarPart {
+    long off;            // element offset of the first row in P / D
+    int nrows, nc;
+    int a, b, c, pad;
+    double xa, yb0, zc0, db, dc;
+};
+
+struct FarFaceIn {
+    const double2 *psi, *dpsi;
+    long ld, j0, k0;
+    int nb, nc;
+    double coef;
+    double2 *P, *D;
+};
+
+__device__ __forceinline__ double nr_weight(int i, int n) {
+    if (n >= 8) {
+        const int e = i < n - 1 - i ? i : n - 1 - i;
+        if (e == 0) return 17.0 / 48.0;
+        if (e == 1) return 59.0 / 48.0;
+        if (e == 2) return 43.0 / 48.0;
+        if (e == 3) return 49.0 / 48.0;
+        return 1.0;
+    }
+    return (n >= 2 && (i == 0 || i == n - 1)) ? 0.5 : 1.0;
+}
+
+// Clip one face to the box, fold in the quadrature weights, the normal's sign
+// and the constant: P = c w_j w_k A, D = c w_j w_k dA/dx_a.
+__global__ __launch_bounds__(256) void k_ff_prep(FarFaceIn F) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)F.nb * F.nc) return;
+    const int j = (int)(i / F.nc), k = (int)(i % F.nc);
+    const double w = F.coef * nr_weight(j, F.nb) * nr_weight(k, F.nc);
+    const long src = (F.j0 + j) * F.ld + F.k0 + k;
+    const double2 p = F.psi[src], q = F.dpsi[src];
+    F.P[i] = make_double2(w * p.x, w * p.y);
+    F.D[i] = make_double2(w * q.x, w * q.y);
+}
+
+// T <- T z + (D + i r_a P)
+__device__ __forceinline__ void horner_step(double &tx, double &ty, double zc, double zs, double ra, double2 p,
+                                            double2 q) {
+    const double gx = fma(-ra, p.y, q.x), gy = fma(ra, p.x, q.y);
+    const double nx = fma(tx, zc, fma(-ty, zs, gx));
+    const double ny = fma(tx, zs, fma(ty, zc, gy));
+    tx = nx;
+    ty = ny;
+}
+
+__device__ __forceinline__ double pick(int a, double x, double y, double z) {
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+
+// One block per (64 R directions, run of rows); lane l of each wave takes
+// directions l, l + 64, ..., l + 64 (R - 1) of the block, and the
+// FF_BLOCK_WAVES waves split the run's rows into consecutive slices, summed
+// through LDS at the end in wave order (deterministic).  Rows are summed from
+// the last to the first (Horner in u), each row's points from the last column
+// to the first (Horner in z).  The surface values arrive in scalar
+// registers; D + i r_a P takes two of them per FMA, so they are copied to
+// vector registers once per point and shared by the lane's R directions.
+template <int R>
+__global__ __launch_bounds__(FF_WAVE *FF_BLOCK_WAVES) void k_far_field(const FarPart *__restrict__ parts,
+                                                                        const double2 *__restrict__ P,
+                                                                        const double2 *__restrict__ D,
+                                                                        const double *__restrict__ dirs, int ndir,
+                                                                        double2 *__restrict__ partial) {
+    __shared__ double2 red[FF_BLOCK_WAVES - 1][R][FF_WAVE];
+    const FarPart pt = parts[blockIdx.y];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / FF_WAVE), lane = threadIdx.x % FF_WAVE;
+    const int slice = (pt.nrows + FF_BLOCK_WAVES - 1) / FF_BLOCK_WAVES;
+    const int j0 = min(pt.nrows, wave * slice), j1 = min(pt.nrows, j0 + slice);
+    double ra[R], zc[R], zs[R], uc[R], us[R], ax[R], ay[R], rbv[R], rcv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int d = (blockIdx.x * R + r) * FF_WAVE + lane;
+        const int dd = d < ndir ? d : ndir - 1;
+        const double r0 = dirs[3 * dd], r1 = dirs[3 * dd + 1], r2 = dirs[3 * dd + 2];
+        ra[r] = pick(pt.a, r0, r1, r2);
+        rbv[r] = pick(pt.b, r0, r1, r2);
+        rcv[r] = pick(pt.c, r0, r1, r2);
+        sincos(rcv[r] * pt.dc, &zs[r], &zc[r]);
+        sincos(rbv[r] * pt.db, &us[r], &uc[r]);
+        zs[r] = -zs[r];   // z = e^{-i r_c d_c}
+        us[r] = -us[r];   // u = e^{-i r_b d_b}
+        ax[r] = 0.0;
+        ay[r] = 0.0;
+    }
+    for (int j = j1 - 1; j >= j0; --j) {
+        const long o = pt.off + (long)j * pt.nc;
+        double tx[R], ty[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) tx[r] = ty[r] = 0.0;
+        int k = pt.nc - 1;
+        // groups of FF_GROUP points: all loads first, one wait, then the steps
+        for (; k >= FF_GROUP - 1; k -= FF_GROUP) {
+            double2 p[FF_GROUP], q[FF_GROUP];
+#pragma unroll
+            for (int u = 0; u < FF_GROUP; ++u) {
+                p[u] = P[o + k - u];
+                q[u] = D[o + k - u];
+            }
+#pragma unroll
+            for (int u = 0; u < FF_GROUP; ++u)
+#pragma unroll
+                for (int r = 0; r < R; ++r) horner_step(tx[r], ty[r], zc[r], zs[r], ra[r], p[u], q[u]);
+        }
+        for (; k >= 0; --k) {
+            const double2 p = P[o + k], q = D[o + k];
+#pragma unroll
+            for (int r = 0; r < R; ++r) horner_step(tx[r], ty[r], zc[r], zs[r], ra[r], p, q);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double nx = fma(ax[r], uc[r], fma(-ay[r], us[r], tx[r]));
+            const double ny = fma(ax[r], us[r], fma(ay[r], uc[r], ty[r]));
+            ax[r] = nx;
+            ay[r] = ny;
+        }
+    }
+    // the slice's phase: its first row is j0
+    const double yb = pt.yb0 + (double)j0 * pt.db;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double ps, pc;
+        sincos(ra[r] * pt.xa + rbv[r] * yb + rcv[r] * pt.zc0, &ps, &pc);
+        ps = -ps;
+        const double fx = ax[r] * pc - ay[r] * ps, fy = ax[r] * ps + ay[r] * pc;
+        ax[r] = fx;
+        ay[r] = fy;
+        if (wave > 0) red[wave - 1][r][lane] = make_double2(fx, fy);
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double fx = ax[r], fy = ay[r];
+#pragma unroll
+            for (int w = 0; w < FF_BLOCK_WAVES - 1; ++w) {
+                fx += red[w][r][lane].x;
+                fy += red[w][r][lane].y;
+            }
+            const int d = (blockIdx.x * R + r) * FF_WAVE + lane;
+            if (d < ndir) partial[(long)blockIdx.y * ndir + d] = make_double2(fx, fy);
+        }
+    }
+}
+
+// Sum of the parts in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_ff_reduce(const double2 *__restrict__ partial, int nparts, int ndir,
+                                                   double *__restrict__ out) {
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= ndir) return;
+    double x = 0.0, y = 0.0;
+    for (int p = 0; p < nparts; ++p) {
+        const double2 v = partial[(long)p * ndir + d];
+        x += v.x;
+        y += v.y;
+    }
+    out[2 * d] = x;
+    out[2 * d + 1] = y;
+}
+
+// Device buffers released on every return path.
+struct DevBufs {
+    std::vector<void *> ptrs;
+    ~DevBufs() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t get(T *&p, size_t bytes) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+        if (e == hipSuccess) ptrs.push_back(q);
+        p = (T *)q;
+        return e;
+    }
+};
+
+const int FACE_AXES[6][3] = {{0, 1, 2}, {0, 1, 2}, {1, 0, 2}, {1, 0, 2}, {2, 0, 1}, {2, 0, 1}};
+
+}  // namespace
+
+namespace pstd {
+
+void divide_complex(double *f, long n, double re, double im) {
+    const double m = re * re + im * im, ir = re / m, ii = -im / m;
+    for (long i = 0; i < n; ++i) {
+        const double x = f[2 * i], y = f[2 * i + 1];
+        f[2 * i] = x * ir - y * ii;
+        f[2 * i + 1] = x * ii + y * ir;
+    }
+}
+
+int far_field_device(const FarGeom &g, const FarPlane planes[6], double scale, long ndir, const double *dirs,
+                     double *f_out, hipStream_t st) {
+    if (ndir <= 0 || ndir > INT_MAX / 4 || !dirs || !f_out) return report_error(PSTD_EINVAL, "bad direction list");
+    for (long i = 0; i < ndir; ++i) {
+        const double n = std::sqrt(dirs[3 * i] * dirs[3 * i] + dirs[3 * i + 1] * dirs[3 * i + 1] +
+                                   dirs[3 * i + 2] * dirs[3 * i + 2]);
+        if (!(std::fabs(n - 1.0) <= 1e-6)) return report_error(PSTD_EINVAL, "directions must be unit vectors");
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!(g.vr[2 * a] >= 0 && g.vr[2 * a] < g.vr[2 * a + 1] && g.vr[2 * a + 1] < g.N[a]))
+            return report_error(PSTD_EINVAL, "virtual surface box outside the grid");
+    long nb[6], nc[6], off[6], total = 0;
+    for (int f = 0; f < 6; ++f) {
+        const int b = FACE_AXES[f][1], c = FACE_AXES[f][2];
+        nb[f] = g.vr[2 * b + 1] - g.vr[2 * b] + 1;
+        nc[f] = g.vr[2 * c + 1] - g.vr[2 * c] + 1;
+        off[f] = total;
+        total += nb[f] * nc[f];
+    }
+    DevBufs buf;
+    double2 *P, *D, *partial;
+    double *ddirs, *dout;
+    FarPart *dparts;
+    FFCHK(buf.get(P, total * sizeof(double2)), "far field buffers");
+    FFCHK(buf.get(D, total * sizeof(double2)), "far field buffers");
+    for (int f = 0; f < 6; ++f) {
+        const int b = FACE_AXES[f][1], c = FACE_AXES[f][2];
+        const double s = (f % 2 == 0) ? -1.0 : 1.0;
+        FarFaceIn F;
+        F.psi = planes[f].psi;
+        F.dpsi = planes[f].dpsi;
+        F.ld = planes[f].ld;
+        F.j0 = g.vr[2 * b];
+        F.k0 = g.vr[2 * c];
+        F.nb = (int)nb[f];
+        F.nc = (int)nc[f];
+        F.coef = -s * scale * g.d[b] * g.d[c] / FOUR_PI;
+        F.P = P + off[f];
+        F.D = D + off[f];
+        const long n = nb[f] * nc[f];
+        hipLaunchKernelGGL(k_ff_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, F);
+        FFCHK(hipGetLastError(), "far field prep");
+    }
+    // runs of rows: enough (direction block, run) waves to fill the device
+    const long dblocks = (ndir + FF_WAVE * FF_DIRS_PER_LANE - 1) / (FF_WAVE * FF_DIRS_PER_LANE);
+    long rows = 0;
+    for (int f = 0; f < 6; ++f) rows += nb[f];
+    const long per = std::max((long)FF_BLOCK_WAVES,
+                              (rows * dblocks * FF_BLOCK_WAVES + FF_TARGET_WAVES - 1) / FF_TARGET_WAVES);
+    std::vector<FarPart> parts;
+    for (int f = 0; f < 6; ++f) {
+        const int a = FACE_AXES[f][0], b = FACE_AXES[f][1], c = FACE_AXES[f][2];
+        for (long j0 = 0; j0 < nb[f]; j0 += per) {
+            FarPart p;
+            p.off = off[f] + j0 * nc[f];
+            p.nrows = (int)std::min(per, nb[f] - j0);
+            p.nc = (int)nc[f];
+            p.a = a;
+            p.b = b;
+            p.c = c;
+            p.pad = 0;
+            p.xa = ((double)g.vr[f] - g.orig[a]) * g.d[a];
+            p.yb0 = ((double)(g.vr[2 * b] + j0) - g.orig[b]) * g.d[b];
+            p.zc0 = ((double)g.vr[2 * c] - g.orig[c]) * g.d[c];
+            p.db = g.d[b];
+            p.dc = g.d[c];
+            parts.push_back(p);
+        }
+    }
+    const long nparts = (long)parts.size();
+    if (nparts > 65535) return report_error(PSTD_EINVAL, "virtual surface too large");
+    FFCHK(buf.get(dparts, nparts * sizeof(FarPart)), "far field buffers");
+    FFCHK(buf.get(ddirs, ndir * 3 * sizeof(double)), "far field buffers");
+    FFCHK(buf.get(partial, nparts * ndir * sizeof(double2)), "far field buffers");
+    FFCHK(buf.get(dout, ndir * 2 * sizeof(double)), "far field buffers");
+    FFCHK(hipMemcpyAsync(dparts, parts.data(), nparts * sizeof(FarPart), hipMemcpyHostToDevice, st), "upload");
+    FFCHK(hipMemcpyAsync(ddirs, dirs, ndir * 3 * sizeof(double), hipMemcpyHostToDevice, st), "upload");
+    hipLaunchKernelGGL(k_far_field<FF_DIRS_PER_LANE>, dim3((unsigned)dblocks, (unsigned)nparts),
+                       dim3(FF_WAVE * FF_BLOCK_WAVES), 0, st, dparts, P, D, ddirs, (int)ndir, partial);
+    FFCHK(hipGetLastError(), "far field");
+    hipLaunchKernelGGL(k_ff_reduce, dim3((unsigned)((ndir + 255) / 256)), dim3(256), 0, st, partial, (int)nparts,
+                       (int)ndir, dout);
+    FFCHK(hipGetLastError(), "far field reduce");
+    FFCHK(hipMemcpyAsync(f_out, dout, ndir * 2 * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+    FFCHK(hipStreamSynchronize(st), "far field sync");
+    return 0;
+}
+
+}  // namespace pstd
+
+extern "C" int pstd_far_field_file(const char *path, const double inc[2], long ndir, const double *dirs,
+                                   double *f_out) {
+    using pstd::report_error;
+    if (!path) return report_error(PSTD_EINVAL, "null path");
+    if (inc && !(inc[0] * inc[0] + inc[1] * inc[1] > 0.0)) return report_error(PSTD_EINVAL, "zero incident amplitude");
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return report_error(PSTD_EIO, std::string("Cannot open ") + path);
+    std::vector<char> raw;
+    {
+        char chunk[1 << 16];
+        size_t n;
+        while ((n = std::fread(chunk, 1, sizeof chunk, fp)) > 0) raw.insert(raw.end(), chunk, chunk + n);
+        std::fclose(fp);
+    }
+    // header, PSTD_QScat.cpp:2406-2420: int precision; long N[3], vrtl[6], nABC; real d[3], orig[3], E0
+    int prec = 0;
+    if (raw.size() < 4) return report_error(PSTD_EIO, "truncated surface file");
+    std::memcpy(&prec, raw.data(), 4);
+    if (prec != 4 && prec != 8) return report_error(PSTD_EIO, "surface file precision must be 4 or 8");
+    const size_t hbytes = 4 + 10 * sizeof(long) + 7 * (size_t)prec;
+    if (raw.size() < hbytes) return report_error(PSTD_EIO, "truncated surface file");
+    long L[10];
+    std::memcpy(L, raw.data() + 4, sizeof L);
+    double R[7];
+    for (int i = 0; i < 7; ++i) {
+        if (prec == 8) {
+            std::memcpy(&R[i], raw.data() + 84 + 8 * i, 8);
+        } else {
+            float v;
+            std::memcpy(&v, raw.data() + 84 + 4 * i, 4);
+            R[i] = v;
+        }
+    }
+    pstd::FarGeom g;
+    for (int a = 0; a < 3; ++a) {
+        g.N[a] = L[a];
+        g.d[a] = R[a];
+        g.orig[a] = R[3 + a];
+        if (L[a] < 1 || L[a] > (1L << 20)) return report_error(PSTD_EIO, "bad grid size in surface file");
+    }
+    for (int i = 0; i < 6; ++i) g.vr[i] = L[3 + i];
+    const long plane[3] = {g.N[1] * g.N[2], g.N[0] * g.N[2], g.N[0] * g.N[1]};
+    const long nreal = 2 * 2 * 2 * (plane[0] + plane[1] + plane[2]);
+    if ((long)(raw.size() - hbytes) != nreal * prec) return report_error(PSTD_EIO, "surface file body size mismatch");
+    std::vector<double> body(nreal);
+    if (prec == 8) {
+        std::memcpy(body.data(), raw.data() + hbytes, nreal * 8);
+    } else {
+        for (long i = 0; i < nreal; ++i) {
+            float v;
+            std::memcpy(&v, raw.data() + hbytes + 4 * i, 4);
+            body[i] = v;
+        }
+    }
+    raw.clear();
+    raw.shrink_to_fit();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return report_error(PSTD_ENODEV, "no HIP device");
+    hipStream_t st;
+    FFCHK(hipStreamCreate(&st), "stream");
+    double2 *dbody = nullptr;
+    hipError_t e = hipMalloc((void **)&dbody, nreal * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        return pstd::report_hip_error(e, "surface upload");
+    }
+    int rc = 0;
+    e = hipMemcpy(dbody, body.data(), nreal * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = pstd::report_hip_error(e, "surface upload");
+    } else {
+        // body: per face psi plane then d/dx_a plane, faces x1 x2 y1 y2 z1 z2
+        pstd::FarPlane pl[6];
+        long o = 0;
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2;
+            pl[f].psi = dbody + o;
+            pl[f].dpsi = dbody + o + plane[a];
+            pl[f].ld = (a == 2) ? g.N[1] : g.N[2];
+            o += 2 * plane[a];
+        }
+        rc = pstd::far_field_device(g, pl, 1.0, ndir, dirs, f_out, st);
+        if (!rc && inc) pstd::divide_complex(f_out, ndir, inc[0], inc[1]);
+    }
+    (void)hipFree(dbody);
+    (void)hipStreamDestroy(st);
+    return rc;
+}

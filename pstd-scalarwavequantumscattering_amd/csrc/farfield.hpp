@@ -1,0 +1,34 @@
+// Far-field transformation of the virtual-surface data on the GPU
+// (farfield.hip).  Shared by the solver's entry point (its on-device
+// accumulators) and the file entry point (a surface file read on the host).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace pstd {
+
+// Geometry of a surface file header (PSTD_QScat.cpp:2406-2420).
+struct FarGeom {
+    long N[3];
+    long vr[6];        // X1V X2V Y1V Y2V Z1V Z2V
+    double d[3], orig[3];
+};
+
+// One face's data on the device: full global planes [N_b][N_c] of psi and of
+// its derivative along +axis, row stride ld (in double2).
+struct FarPlane {
+    const double2 *psi, *dpsi;
+    long ld;
+};
+
+// f(rhat) for ndir unit vectors (host, [ndir][3]) into f_out (host,
+// [ndir][2]).  scale multiplies the plane values (1/steps for raw running
+// sums, 1 for normalised data).  Work is queued on st and waited for.
+int far_field_device(const FarGeom &g, const FarPlane planes[6], double scale, long ndir, const double *dirs,
+                     double *f_out, hipStream_t st);
+
+// f[i] /= (re + i im) for n interleaved complex values.
+void divide_complex(double *f, long n, double re, double im);
+
+}  // namespace pstd

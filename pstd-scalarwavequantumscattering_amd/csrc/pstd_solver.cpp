@@ -12,6 +12,8 @@
 #include <vector>
 
 #include "pstd_host.hpp"
+#include "../../include/pstd_farfield.h"
+#include "farfield.hpp"
 #include "pstd_kernels.hpp"
 
 using namespace pstd;
@@ -482,6 +484,15 @@ int fetch_acc(pstd_solver *s, std::vector<double2> &acc) {
     return 0;
 }
 
+// The incident wave e^{i khat.(r - r_inj)} (PSTD_QScat.cpp:624-653, phase
+// per axis (i - i0) d khat_a as at :983-990) evaluated at the grid centre.
+void incident_amplitude(const pstd_params &p, const Geometry &g, double out[2]) {
+    double ph = 0.0;
+    for (int a = 0; a < 3; ++a) ph += (g.orig[a] - (double)g.inj[a]) * p.d[a] * g.khat[a];
+    out[0] = std::cos(ph);
+    out[1] = std::sin(ph);
+}
+
 int write_surface_file(pstd_solver *s, const std::string &name, const std::vector<double> &body) {
     FILE *fp = std::fopen(name.c_str(), "wb");
     if (!fp) return fail(PSTD_EIO, "Cannot open output file " + name);
@@ -591,6 +602,42 @@ int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix) {
     unpack_body(s, PSTD_DERIV_FD, acc, body.data());
     if ((rc = write_surface_file(s, std::string(prefix) + "_FD_" + std::to_string(n0) + ".dat", body))) return rc;
     return pstd_reset_surface_terms(s);
+}
+
+int pstd_far_field(pstd_solver *s, int deriv, long ndir, const double *dirs, double *f_out) {
+    if (!s || (deriv != PSTD_DERIV_PS && deriv != PSTD_DERIV_FD)) return fail(PSTD_EINVAL, "bad argument");
+    if (s->nSur <= 0) return fail(PSTD_EINVAL, "no surface terms accumulated");
+    const Geometry &g = s->g;
+    pstd::FarGeom fg;
+    for (int a = 0; a < 3; ++a) {
+        fg.N[a] = g.ax[a].N;
+        fg.vr[2 * a] = g.vrtl1[a];
+        fg.vr[2 * a + 1] = g.vrtl2[a];
+        fg.d[a] = s->p.d[a];
+        fg.orig[a] = g.orig[a];
+    }
+    pstd::FarPlane pl[6];
+    for (int f = 0; f < 6; ++f) {   // accumulators per face: psi, PS, FD planes
+        const double2 *base = s->surf_acc + s->plane_off[f];
+        pl[f].psi = base;
+        pl[f].dpsi = base + (deriv == PSTD_DERIV_FD ? 2 : 1) * s->plane_n[f];
+        pl[f].ld = (f / 2 == 2) ? g.ax[1].N : g.ax[2].N;
+    }
+    int rc = pstd::far_field_device(fg, pl, 1.0 / (double)s->nSur, ndir, dirs, f_out, s->stream);
+    if (rc) return rc;
+    double inc[2];
+    incident_amplitude(s->p, g, inc);
+    pstd::divide_complex(f_out, ndir, inc[0], inc[1]);
+    return 0;
+}
+
+int pstd_incident_amplitude(const pstd_params *p, double out[2]) {
+    if (!p || !out) return fail(PSTD_EINVAL, "null argument");
+    Geometry g;
+    int rc = build_geometry(*p, g, g_err);
+    if (rc) return rc;
+    incident_amplitude(*p, g, out);
+    return 0;
 }
 
 // Subdomain rank in MPI's row-major Cartesian order (qscat.cpp:156-157,

@@ -1,0 +1,91 @@
+"""Far-field post-processor over the C ABI of include/pstd_farfield.h.
+
+f(rhat) is the scattering amplitude of the steady state in the solver's
+scaled units (lambdabar = 1, k = 1): A(r) -> f(rhat) e^{i r} / r.  The
+transform runs on the GPU from the solver's accumulators or from a
+virtual-surface file; this module marshals arguments and adds the sphere
+quadrature for cross sections.  The differential cross section is |f|^2
+(lambdabar^2); in physical units multiply by lambdabar^2 = 1/k^2.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .params import PSTD_DERIV_PS
+from .solver import PSTDQScat, _check, _dp, library
+
+
+def incident_amplitude(params):
+    """The solver's incident wave at the grid centre, e^{-i khat.r_inj}
+    (its phase origin is the injection corner, PSTD_QScat.cpp:624-653)."""
+    out = np.zeros(2)
+    _check(library().pstd_incident_amplitude(C.byref(params), _dp(out)))
+    return complex(out[0], out[1])
+
+
+def far_field(source, dirs, deriv=PSTD_DERIV_PS, params=None, inc=None):
+    """f at the unit vectors dirs [n][3].  source: a PSTDQScat (its current
+    accumulation window, derivative kind deriv; normalised to its own
+    incident wave) or the path of a virtual-surface file (which carries its
+    own derivative kind), normalised by the incident amplitude inc at the
+    grid centre, or by that of the run's params; neither: the transform of
+    the data as stored."""
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(d), 2))
+    L = library()
+    if isinstance(source, PSTDQScat):
+        _check(L.pstd_far_field(source.h, int(deriv), len(d), _dp(d), _dp(out)))
+    else:
+        if inc is None and params is not None:
+            inc = incident_amplitude(params)
+        ip = None if inc is None else _dp(np.array([complex(inc).real, complex(inc).imag]))
+        _check(L.pstd_far_field_file(str(source).encode(), ip, len(d), _dp(d), _dp(out)))
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def incident_direction(params):
+    """khat of the incident wave (PSTD_QScat.cpp:624-653)."""
+    t, p = params.theta, params.phi
+    return np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
+
+
+def frame(axis):
+    """Orthonormal (e1, e2, axis)."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    t = np.array([1.0, 0.0, 0.0]) if abs(a[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(a, t)
+    e1 /= np.linalg.norm(e1)
+    return e1, np.cross(a, e1), a
+
+
+def directions(axis, cos_theta, phi):
+    """Unit vectors at polar angle acos(cos_theta) from axis and azimuth phi
+    (broadcast together)."""
+    e1, e2, a = frame(axis)
+    ct = np.asarray(cos_theta, dtype=np.float64)
+    ph = np.asarray(phi, dtype=np.float64)
+    ct, ph = np.broadcast_arrays(ct, ph)
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    v = (st * np.cos(ph))[..., None] * e1 + (st * np.sin(ph))[..., None] * e2 + ct[..., None] * a
+    return v.reshape(-1, 3)
+
+
+def sphere_quadrature(n_theta, n_phi, axis=(0.0, 0.0, 1.0)):
+    """Gauss-Legendre in cos(theta) about axis times the periodic trapezoid
+    in phi: (dirs [n][3], weights [n] summing to 4 pi, cos_theta [n], phi [n])."""
+    x, w = np.polynomial.legendre.leggauss(n_theta)
+    ph = 2 * np.pi * np.arange(n_phi) / n_phi
+    X, PH = np.meshgrid(x, ph, indexing="ij")
+    W = np.repeat(w, n_phi) * (2 * np.pi / n_phi)
+    return directions(axis, X.ravel(), PH.ravel()), W, X.ravel(), PH.ravel()
+
+
+def cross_section(f, weights):
+    """Integral of |f|^2 over the sphere with the quadrature weights."""
+    return float(np.sum(np.asarray(weights) * np.abs(f) ** 2))
+
+
+def optical_theorem(f_forward):
+    """sigma_tot = 4 pi Im f(khat) (k = 1)."""
+    return float(4 * np.pi * np.imag(f_forward))
