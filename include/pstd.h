@@ -55,9 +55,18 @@ extern "C" {
 #define PSTD_DERIV_PS 0
 #define PSTD_DERIV_FD 1
 
+/* Transition profile xi(x) of the TF/SF slabs, the initial taper and the
+ * overlap windows (utility.cpp:32-45, selected at compile time by __IBH__
+ * in the reference): xi = A0 x + A1/2pi sin 2pi x + A2/4pi sin 4pi x +
+ * A3/6pi sin 6pi x. */
+#define PSTD_PROFILE_SMOOTH4 0   /* A = 1, -4/3, 1/3, 0 (default build)      */
+#define PSTD_PROFILE_IBH     1   /* integrated Blackman-Harris:
+                                    A = 1, -1.3611, 0.3938, -0.0326        */
+
 /* The ten fields of the reference input file, in file order
- * (qscat.cpp:228-293), plus the precision switch that the reference selects
- * at compile time (run_environment.h:36-78, makefile:7-15). */
+ * (qscat.cpp:228-293), plus the precision and profile switches that the
+ * reference selects at compile time (run_environment.h:36-78,
+ * makefile:7-15; utility.cpp:32-45). */
 typedef struct pstd_params {
     int    nprocs[3];      /* Px Py Pz: subdomain grid                     */
     double dtau;           /* time step, radians of omega*t                */
@@ -74,6 +83,7 @@ typedef struct pstd_params {
     long   iterations;     /* Iter                                         */
     char   prefix[256];    /* surface file name prefix                     */
     int    precision;      /* PSTD_FP64 or PSTD_FP32                       */
+    int    profile;        /* PSTD_PROFILE_SMOOTH4 (0) or PSTD_PROFILE_IBH */
 } pstd_params;
 
 /* Global geometry derived from pstd_params; the index maths of
@@ -172,8 +182,15 @@ int pstd_get_surfaces(pstd_solver *s, int deriv, double *out, size_t nvals);
  * write <prefix>_PS_<n>.dat and <prefix>_FD_<n>.dat with the reference's
  * header (int precision, 10 long, 7 QPrecision) and body, then reset.
  * With no accumulation it prints the reference's warning and writes
- * nothing. */
+ * nothing.  QPrecision is the file precision (pstd_set_file_precision). */
 int pstd_save_virtual_surfaces(pstd_solver *s, const char *prefix);
+
+/* Precision of the surface files, PSTD_FP64 (8: double header reals and
+ * body) or PSTD_FP32 (4: float, as the reference's single-precision build
+ * writes them, PSTD_QScat.cpp:2429-2552 with QPrecision = float).  The
+ * default is the solve's precision.  The accumulators are FP64 in both
+ * solve precisions; a precision-4 file rounds them once, at the write. */
+int pstd_set_file_precision(pstd_solver *s, int precision);
 
 /* Zero the accumulators (ResetSurfaceTerms, PSTD_QScat.cpp:2156-2189). */
 int pstd_reset_surface_terms(pstd_solver *s);

@@ -877,6 +877,10 @@ int validate(const pstd_params *p) {
 Oracle *build(const pstd_params *p, const std::function<double(double, double, double)> &V, int &rc) {
     rc = validate(p);
     if (rc) return nullptr;
+    // the profile of this run (utility.cpp:32-45, __IBH__ at compile time in
+    // the reference); every profile evaluation happens during this build
+    const Profile saved = g_prof;
+    g_prof = (p->profile == 1) ? IBH : SMOOTH4;
     Oracle *O = new Oracle();
     O->p = *p;
     O->alpha2 = 2.0 * std::sin(p->dtau);  // :134
@@ -894,6 +898,7 @@ Oracle *build(const pstd_params *p, const std::function<double(double, double, d
     init_psi(*O);
     init_v(*O, V);
     init_surfaces(*O);
+    g_prof = saved;
     return O;
 }
 

@@ -11,7 +11,8 @@ of the solver fails loudly: there is no CPU fallback.
 from .params import (  # noqa: F401
     MAX_SEG, NWEIGHT, PSTD_DERIV_FD, PSTD_DERIV_PS, PSTD_EINPUT, PSTD_EINVAL,
     PSTD_EIO, PSTD_ENODEV, PSTD_ENOMEM, PSTD_ETOPO, PSTD_FP32, PSTD_FP64,
-    PSTD_METHOD_FFT, PSTD_METHOD_GEMM, PSTD_OK, PstdGeometry, PstdParams, global_size,
+    PSTD_METHOD_FFT, PSTD_METHOD_GEMM, PSTD_OK, PSTD_PROFILE_IBH, PSTD_PROFILE_SMOOTH4, PstdGeometry, PstdParams,
+    global_size,
     make_params, surface_body_len,
 )
 from .solver import (  # noqa: F401

@@ -13,24 +13,30 @@ const double TINY = 1.0e-10;                                   // run_environmen
 const double TWOPI = 6.283185307179586476925286766559005768394;
 const double FOURPI = 12.56637061435917295385057353311801153679;
 const long double PI_L = 3.141592653589793238462643383279502884197L;
-// SMOOTH4 coefficients (utility.cpp:39-45)
-const double A0 = 1.0, A1 = -4.0 / 3.0, A2 = 1.0 / 3.0;
+const double SIXPI = 18.84955592153875943077586029967701730518;
+// A0..A3 of SMOOTH4 (utility.cpp:39-45) and of the integrated
+// Blackman-Harris set (utility.cpp:32-37)
+const double PROF[2][4] = {{1.0, -4.0 / 3.0, 1.0 / 3.0, 0.0}, {1.0, -1.3611, 0.3938, -0.0326}};
 // FD8 coefficients (PSTD_QScat.cpp:43-46)
 const double FDC[4] = {4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0};
 }  // namespace
 
-double ksi(double x) {
+double ksi(double x, int prof) {
     if (x < TINY) return 0.0;
     if (x > 1.0 - TINY) return 1.0;
-    return A0 * x + (A1 / TWOPI) * std::sin(TWOPI * x) + (A2 / FOURPI) * std::sin(FOURPI * x);
+    const double *A = PROF[prof];
+    return A[0] * x + (A[1] / TWOPI) * std::sin(TWOPI * x) + (A[2] / FOURPI) * std::sin(FOURPI * x) +
+           (A[3] / SIXPI) * std::sin(SIXPI * x);
 }
-double dksi(double x) {
+double dksi(double x, int prof) {
     if (x < TINY || x > 1.0 - TINY) return 0.0;
-    return A0 + A1 * std::cos(TWOPI * x) + A2 * std::cos(FOURPI * x);
+    const double *A = PROF[prof];
+    return A[0] + A[1] * std::cos(TWOPI * x) + A[2] * std::cos(FOURPI * x) + A[3] * std::cos(SIXPI * x);
 }
-double d2ksi(double x) {
+double d2ksi(double x, int prof) {
     if (x < TINY || x > 1.0 - TINY) return 0.0;
-    return -TWOPI * (A1 * std::sin(TWOPI * x) + 2.0 * A2 * std::sin(FOURPI * x));
+    const double *A = PROF[prof];
+    return -TWOPI * (A[1] * std::sin(TWOPI * x) + 2.0 * A[2] * std::sin(FOURPI * x) + 3.0 * A[3] * std::sin(SIXPI * x));
 }
 
 static void build_axis(Axis &a, long P, long L, long h, double d) {
@@ -86,6 +92,12 @@ int build_geometry(const pstd_params &p, Geometry &g, std::string &err) {
         err = "precision must be 8 (FP64) or 4 (FP32)";
         return PSTD_EINVAL;
     }
+    if (p.profile != PSTD_PROFILE_SMOOTH4 && p.profile != PSTD_PROFILE_IBH) {
+        err = "profile must be 0 (SMOOTH4) or 1 (integrated Blackman-Harris)";
+        return PSTD_EINVAL;
+    }
+    g.prof = p.profile;
+    for (int a = 0; a < 3; ++a) g.ax[a].prof = p.profile;
     for (int a = 0; a < 3; ++a) {
         long N = g.ax[a].N;
         g.abc1[a] = p.n_abc - 1;
@@ -154,11 +166,12 @@ std::vector<long double> segment_window(const Axis &a, int r) {
     if (a.P > 1) {
         if (r > 0) {
             W[0] = 0.0L;
-            for (int m = 1; m <= PSTD_NWEIGHT; ++m) W[m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1));
+            for (int m = 1; m <= PSTD_NWEIGHT; ++m) W[m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1), a.prof);
         }
         if (r < a.P - 1) {
             W[a.L - 1] = 0.0L;
-            for (int m = 1; m <= PSTD_NWEIGHT; ++m) W[a.L - 1 - m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1));
+            for (int m = 1; m <= PSTD_NWEIGHT; ++m)
+                W[a.L - 1 - m] = ksi((double)m / (double)(PSTD_NWEIGHT + 1), a.prof);
         }
     }
     return W;
@@ -329,10 +342,12 @@ TfsfAxis tfsf_axis(const Geometry &g, int a, double dtau, long n_ksi) {
         // [ksi1a, ksi2b]; the slab-normal coefficient on the slab ranges
         // [ksi1a, ksi1b] and [ksi2a, ksi2b], which reach past the box when
         // the box is thinner than two transition layers (:741-771)
-        if (i >= g.ksi1a[a] && i <= g.ksi2b[a]) t.P[i] = ksi(s1) * ksi(s2);
+        if (i >= g.ksi1a[a] && i <= g.ksi2b[a]) t.P[i] = ksi(s1, g.prof) * ksi(s2, g.prof);
         std::complex<double> A = 0.0;
-        if (i >= g.ksi1a[a] && i <= g.ksi1b[a]) A += coef * std::complex<double>(d2ksi(s1) / l, 2.0 * dksi(s1) * kh);
-        if (i >= g.ksi2a[a] && i <= g.ksi2b[a]) A += coef * std::complex<double>(d2ksi(s2) / l, -2.0 * dksi(s2) * kh);
+        if (i >= g.ksi1a[a] && i <= g.ksi1b[a])
+            A += coef * std::complex<double>(d2ksi(s1, g.prof) / l, 2.0 * dksi(s1, g.prof) * kh);
+        if (i >= g.ksi2a[a] && i <= g.ksi2b[a])
+            A += coef * std::complex<double>(d2ksi(s2, g.prof) / l, -2.0 * dksi(s2, g.prof) * kh);
         t.A[i] = A;
     }
     return t;
@@ -351,8 +366,8 @@ void initial_field(const Geometry &g, const pstd_params &p, std::vector<double> 
         prof[a].assign(g.ax[a].N, 0.0);
         for (long i = g.ksi1a[a]; i <= g.ksi2b[a]; ++i) {
             double t = 1.0;
-            if (i <= g.ksi1b[a]) t *= ksi((double)(i - g.ksi1a[a]) / n0t1);
-            if (i >= g.ksi2a[a]) t *= ksi((double)(g.ksi2b[a] - i) / n0t1);
+            if (i <= g.ksi1b[a]) t *= ksi((double)(i - g.ksi1a[a]) / n0t1, g.prof);
+            if (i >= g.ksi2a[a]) t *= ksi((double)(g.ksi2b[a] - i) / n0t1, g.prof);
             prof[a][i] = t;
         }
     }

@@ -19,6 +19,7 @@ namespace pstd {
 struct Axis {
     long N = 0, L = 0, P = 1, h = 0;
     double d = 0.0;
+    int prof = PSTD_PROFILE_SMOOTH4;        // window profile
     std::vector<long> fft1, core1, core2;   // global, inclusive
     long ncore(int r) const { return core2[r] - core1[r] + 1; }
     long i1(int r) const { return core1[r] - fft1[r]; }   // local core start
@@ -35,6 +36,7 @@ struct Geometry {
     long vrtl1[3], vrtl2[3], vbox1[3], vbox2[3], inj[3];
     double orig[3];
     double khat[3];          // sin t cos p, sin t sin p, cos t
+    int prof = PSTD_PROFILE_SMOOTH4;        // transition profile
 };
 
 // Validation + decomposition.  Returns a pstd error code, message in err.
@@ -42,9 +44,9 @@ int build_geometry(const pstd_params &p, Geometry &g, std::string &err);
 void export_geometry(const Geometry &g, pstd_geometry *out);
 
 // Smooth profile family (utility.cpp:47-106), FP64 with the TINY clamp.
-double ksi(double x);
-double dksi(double x);
-double d2ksi(double x);
+double ksi(double x, int prof);
+double dksi(double x, int prof);
+double d2ksi(double x, int prof);
 
 // Window of segment r along an axis: plane 0 -> 0 and planes 1..8 ->
 // xi(m/9) on a side that has a neighbour (PSTD_QScat.cpp:104-105, :1554-1603).

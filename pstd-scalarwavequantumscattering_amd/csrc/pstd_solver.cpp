@@ -83,6 +83,7 @@ struct pstd_solver {
     SurfaceOp sop[6];
     long plane_off[6], plane_n[6];
     long nTau = 1, nSur = 0;
+    int file_prec = PSTD_FP64;    // surface-file precision (header reals and body)
     std::vector<void *> allocs;
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -167,6 +168,7 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
         return fail(PSTD_EINVAL, "vbox has " + std::to_string(nv) + " values, potential box needs " + std::to_string(nvb));
     }
     s->esz = (p->precision == PSTD_FP32) ? 4 : 8;
+    s->file_prec = p->precision;
     if ((rc = select_device(s))) { delete s; return rc; }
     const long double alpha = 2.0L * (long double)std::sin(p->dtau);   // PSTD_QScat.cpp:134
     for (int a = 0; a < 3; ++a) {
@@ -497,13 +499,21 @@ int write_surface_file(pstd_solver *s, const std::string &name, const std::vecto
     FILE *fp = std::fopen(name.c_str(), "wb");
     if (!fp) return fail(PSTD_EIO, "Cannot open output file " + name);
     const Geometry &g = s->g;
-    int precision = 8;   // accumulators and file body are FP64 in both modes
+    const int precision = s->file_prec;   // sizeof(QPrecision), PSTD_QScat.cpp:2432
     long hdr[10] = {g.ax[0].N, g.ax[1].N, g.ax[2].N, g.vrtl1[0], g.vrtl2[0],
                     g.vrtl1[1], g.vrtl2[1], g.vrtl1[2], g.vrtl2[2], s->p.n_abc};
     double dh[7] = {s->p.d[0], s->p.d[1], s->p.d[2], g.orig[0], g.orig[1], g.orig[2], s->p.E0};
-    bool ok = std::fwrite(&precision, sizeof(int), 1, fp) == 1 && std::fwrite(hdr, sizeof(long), 10, fp) == 10 &&
-              std::fwrite(dh, sizeof(double), 7, fp) == 7 &&
-              std::fwrite(body.data(), sizeof(double), body.size(), fp) == body.size();
+    bool ok = std::fwrite(&precision, sizeof(int), 1, fp) == 1 && std::fwrite(hdr, sizeof(long), 10, fp) == 10;
+    if (precision == PSTD_FP64) {
+        ok = ok && std::fwrite(dh, sizeof(double), 7, fp) == 7 &&
+             std::fwrite(body.data(), sizeof(double), body.size(), fp) == body.size();
+    } else {
+        float fh[7];
+        for (int i = 0; i < 7; ++i) fh[i] = (float)dh[i];
+        std::vector<float> fb(body.begin(), body.end());
+        ok = ok && std::fwrite(fh, sizeof(float), 7, fp) == 7 &&
+             std::fwrite(fb.data(), sizeof(float), fb.size(), fp) == fb.size();
+    }
     ok = (std::fclose(fp) == 0) && ok;
     if (!ok) return fail(PSTD_EIO, "write failed: " + name);
     return 0;
@@ -770,6 +780,13 @@ int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long 
     }
     *periods_out = k;
     *converged_out = conv ? 1 : 0;
+    return 0;
+}
+
+int pstd_set_file_precision(pstd_solver *s, int precision) {
+    if (!s || (precision != PSTD_FP64 && precision != PSTD_FP32))
+        return fail(PSTD_EINVAL, "file precision must be 8 (FP64) or 4 (FP32)");
+    s->file_prec = precision;
     return 0;
 }
 

@@ -2,8 +2,12 @@
 // MI355X library.  One process, one GPU; every subdomain of the input's
 // process grid runs on the device as a segment of one global grid.
 //
-//   qscat_amd -i FILE [--fp32] [--no-planes]
+//   qscat_amd -i FILE [--fp32] [--ibh] [--no-planes]
 //
+// --fp32 and --ibh stand for the reference's compile-time switches (the
+// fqscat build, QPrecision = float, and __IBH__, the integrated
+// Blackman-Harris profile of utility.cpp:32-37); an FP32 run writes
+// precision-4 surface files as fqscat does.
 // The time loop, the one-period accumulation windows (the last period, and
 // the periods ending at steps 20000/40000/60000) and the surface files are
 // those of the reference (qscat.cpp:165-198).  So are the diagnostics: the
@@ -22,12 +26,12 @@
 extern "C" double pstd_default_potential(double x, double y, double z);
 
 static void show_usage(const char *app) {
-    std::printf("Usage: %s -i FILE [--fp32] [--no-planes]\nwhere FILE: input file for model parameters\n", app);
+    std::printf("Usage: %s -i FILE [--fp32] [--ibh] [--no-planes]\nwhere FILE: input file for model parameters\n", app);
 }
 
 int main(int argc, char **argv) {
     std::string openname;
-    bool fp32 = false, planes = true;
+    bool fp32 = false, ibh = false, planes = true;
     for (int i = 1; i < argc; ++i) {
         std::string arg = argv[i];
         if (arg == "-h" || arg == "--help") {
@@ -36,6 +40,7 @@ int main(int argc, char **argv) {
         }
         if (arg == "--fp32") fp32 = true;
         if (arg == "--no-planes") planes = false;
+        if (arg == "--ibh") ibh = true;
         if (arg == "-i") {
             if (i < argc - 1) openname = argv[i + 1];
             if (i == argc - 1 || openname[0] == '-') {
@@ -57,6 +62,7 @@ int main(int argc, char **argv) {
         return 1;
     }
     if (fp32) p.precision = PSTD_FP32;
+    if (ibh) p.profile = PSTD_PROFILE_IBH;
     pstd_solver *s = nullptr;
     rc = pstd_create(&p, pstd_default_potential, &s);
     if (rc) {

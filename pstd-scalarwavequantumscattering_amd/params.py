@@ -1,8 +1,8 @@
 """ctypes mirrors of the C structs in include/pstd.h.
 
 ``PstdParams`` carries the ten fields of the reference input file
-(qscat.cpp:228-293) plus the precision switch the reference selects at
-compile time (run_environment.h:36-78).  ``PstdGeometry`` is the
+(qscat.cpp:228-293) plus the precision and profile switches the reference
+selects at compile time (run_environment.h:36-78, utility.cpp:32-45).  ``PstdGeometry`` is the
 decomposition of CPSTD_QScat::DomainDecomposition (PSTD_QScat.cpp:270-396).
 """
 import ctypes as C
@@ -17,6 +17,8 @@ PSTD_ENODEV = 19
 
 PSTD_FP64 = 8
 PSTD_FP32 = 4
+PSTD_PROFILE_SMOOTH4 = 0
+PSTD_PROFILE_IBH = 1
 PSTD_METHOD_GEMM = 0
 PSTD_METHOD_FFT = 1
 PSTD_DERIV_PS = 0
@@ -43,6 +45,7 @@ class PstdParams(C.Structure):
         ("iterations", C.c_long),
         ("prefix", C.c_char * 256),
         ("precision", C.c_int),
+        ("profile", C.c_int),
     ]
 
     def as_dict(self):
@@ -52,7 +55,7 @@ class PstdParams(C.Structure):
             "n_ksi": self.n_ksi, "n_halo": self.n_halo, "theta": self.theta,
             "phi": self.phi, "E0": self.E0, "cap_U0": self.cap_U0,
             "cap_alpha": self.cap_alpha, "iterations": self.iterations,
-            "prefix": self.prefix.decode(), "precision": self.precision,
+            "prefix": self.prefix.decode(), "precision": self.precision, "profile": self.profile,
         }
 
 
@@ -74,7 +77,7 @@ class PstdGeometry(C.Structure):
 
 def make_params(nprocs, dtau, d, L, n_abc, n_sca, n_ksi, n_halo, theta, phi,
                 E0, cap_U0, cap_alpha, iterations=0, prefix="surf",
-                precision=PSTD_FP64):
+                precision=PSTD_FP64, profile=PSTD_PROFILE_SMOOTH4):
     """Build a PstdParams from the ten input-file fields (qscat.cpp:228-293)."""
     p = PstdParams()
     p.nprocs[:] = [int(v) for v in nprocs]
@@ -87,6 +90,7 @@ def make_params(nprocs, dtau, d, L, n_abc, n_sca, n_ksi, n_halo, theta, phi,
     p.iterations = int(iterations)
     p.prefix = prefix.encode()[:255]
     p.precision = int(precision)
+    p.profile = int(profile)
     return p
 
 

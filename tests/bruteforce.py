@@ -12,10 +12,17 @@ import math
 import numpy as np
 
 
-def xi(x):
-    """SMOOTH4 profile, closed form (utility.cpp:47-58)."""
+# A1, A2, A3 of SMOOTH4 (profile 0) and of the integrated Blackman-Harris set
+# (profile 1), utility.cpp:32-45
+PROFILES = [(-4.0 / 3.0, 1.0 / 3.0, 0.0), (-1.3611, 0.3938, -0.0326)]
+
+
+def xi(x, prof=0):
+    """Transition profile, closed form (utility.cpp:47-58)."""
+    a1, a2, a3 = PROFILES[prof]
     x = np.asarray(x, dtype=np.float64)
-    v = x - (4.0 / 3.0) / (2 * math.pi) * np.sin(2 * math.pi * x) + (1.0 / 3.0) / (4 * math.pi) * np.sin(4 * math.pi * x)
+    t = 2 * math.pi * x
+    v = x + a1 / (2 * math.pi) * np.sin(t) + a2 / (4 * math.pi) * np.sin(2 * t) + a3 / (6 * math.pi) * np.sin(3 * t)
     return np.where(x < 1e-10, 0.0, np.where(x > 1 - 1e-10, 1.0, v))
 
 
@@ -33,9 +40,9 @@ def segments(P, L, h):
     return out
 
 
-def window(P, L, r):
+def window(P, L, r, prof=0):
     W = np.ones(L)
-    w = xi(np.arange(1, 9) / 9.0)
+    w = xi(np.arange(1, 9) / 9.0, prof)
     if r > 0:
         W[0] = 0.0
         W[1:9] = w
@@ -74,7 +81,7 @@ def laplacian_term(psi, p):
         x = np.moveaxis(psi, a, 0)
         o = np.moveaxis(out, a, 0)
         for r, (f1, c1, c2) in enumerate(segments(P, L, h)):
-            seg = x[f1:f1 + L] * window(P, L, r)[:, None, None]
+            seg = x[f1:f1 + L] * window(P, L, r, p.profile)[:, None, None]
             u = np.tensordot(D, seg, axes=(1, 0))
             o[c1:c2 + 1] += u[c1 - f1:c2 - f1 + 1]
     return out

@@ -164,3 +164,61 @@ def test_well96_fp64_parity():
     g.update_accumulate(30)
     for deriv in (0, 1):
         assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 1e-10
+
+
+@pytest.mark.parametrize("cfg", CASES[:2])
+def test_fp64_ibh_profile(cfg):
+    """The integrated Blackman-Harris profile (the reference's __IBH__ build,
+    utility.cpp:32-37): overlap windows, TF/SF slabs and the initial taper
+    switch together; same bars as the default profile."""
+    p, V = W.config_tiny(**cfg)
+    p.profile = pstd_amd.PSTD_PROFILE_IBH
+    print(f"case IBH nprocs={tuple(p.nprocs)} L={tuple(p.L)} h={p.n_halo}", flush=True)
+    o, g = O.Oracle(p, V), pstd_amd.PSTDQScat(p, V)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-14
+    o.update(30)
+    g.Update(30)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-10
+    o.update_accumulate(8)
+    g.update_accumulate(8)
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < 1e-10
+    p0, _ = W.config_tiny(**cfg)
+    g0 = pstd_amd.PSTDQScat(p0, V)
+    g0.Update(30)
+    g0.update_accumulate(8)
+    assert rel(g0.get_psi(0), g.get_psi(0)) > 1e-6          # the profile does change the run
+    g.close()
+    g0.close()
+
+
+def test_fp32_surface_files_are_float(tmp_path):
+    """An FP32 solve writes precision-4 files (float header reals and body),
+    as the reference's single-precision build does; the body is the FP64
+    accumulators rounded once.  pstd_set_file_precision(8) keeps FP64."""
+    from pstd_amd.surface_io import read_surface_file
+    p, V = W.config_tiny(precision=pstd_amd.PSTD_FP32, **CASES[0])
+    print("case FP32 surface files", flush=True)
+    g = pstd_amd.PSTDQScat(p, V)
+    g.Update(10)
+    g.update_accumulate(6)
+    want = {d: g.get_surfaces(d) for d in (0, 1)}
+    n0 = g.time_index() - 6
+    g.SaveVirtualSurfaces(str(tmp_path / "f"))
+    for d, kind in ((0, "PS"), (1, "FD")):
+        raw = (tmp_path / f"f_{kind}_{n0}.dat").read_bytes()
+        assert len(raw) == 4 + 80 + 7 * 4 + 4 * want[d].size
+        hdr, body = read_surface_file(tmp_path / f"f_{kind}_{n0}.dat")
+        assert hdr["precision"] == 4 and body.dtype == np.float32
+        assert hdr["d"] == tuple(float(np.float32(v)) for v in p.d)
+        assert np.array_equal(body, want[d].astype(np.float32))
+    g.update_accumulate(6)
+    want = g.get_surfaces(0)
+    n0 = g.time_index() - 6
+    g.set_file_precision(pstd_amd.PSTD_FP64)
+    g.SaveVirtualSurfaces(str(tmp_path / "d"))
+    hdr, body = read_surface_file(tmp_path / f"d_PS_{n0}.dat")
+    assert hdr["precision"] == 8 and np.array_equal(body, want)
+    with pytest.raises(pstd_amd.PSTDError):
+        g.set_file_precision(2)
+    g.close()

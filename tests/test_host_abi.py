@@ -104,20 +104,27 @@ def test_geometry_rejects_bad_inputs():
     with pytest.raises(pstd_amd.PSTDError) as e:
         pstd_amd.compute_geometry(p)
     assert e.value.code == pstd_amd.PSTD_EINVAL
+    p, _ = W.config_tiny(well=None)
+    p.profile = 2
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.compute_geometry(p)
+    assert e.value.code == pstd_amd.PSTD_EINVAL
 
 
 @pytest.mark.parametrize("cfg", [dict(nprocs=(3, 1, 2), L=(40, 24, 30), h=9),
                                  dict(nprocs=(4, 4, 4), L=(80, 80, 80), h=16, n_abc=14, n_sca=7, n_ksi=12)])
-def test_laplacian_operator_tables(cfg):
+@pytest.mark.parametrize("prof", [pstd_amd.PSTD_PROFILE_SMOOTH4, pstd_amd.PSTD_PROFILE_IBH])
+def test_laplacian_operator_tables(cfg, prof):
     """The kernels' alpha * D2 * W rows equal the dense restatement built from
-    numpy's FFT of unit vectors (1e-13 relative)."""
+    numpy's FFT of unit vectors (1e-13 relative), for either window profile."""
     p, _ = W.config_tiny(well=None, **cfg)
+    p.profile = prof
     alpha = 2 * math.sin(p.dtau)
     for a in range(3):
         D = BF.d2_matrix(p.L[a], p.d[a])
         for r, (f1, c1, c2) in enumerate(BF.segments(p.nprocs[a], p.L[a], p.n_halo)):
             A = pstd_amd.line_operator(p, a, r, 0)
-            ref = alpha * D[c1 - f1:c2 - f1 + 1] * BF.window(p.nprocs[a], p.L[a], r)[None, :]
+            ref = alpha * D[c1 - f1:c2 - f1 + 1] * BF.window(p.nprocs[a], p.L[a], r, prof)[None, :]
             assert A.shape == ref.shape
             assert np.abs(A - ref).max() <= 1e-13 * np.abs(ref).max()
 

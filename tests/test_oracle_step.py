@@ -10,19 +10,20 @@ from oracle import oracle as O
 import bruteforce as BF
 
 
-def _xi_np(s):
-    return BF.xi(s)
+PROFILES = [0, 1]   # SMOOTH4 and the integrated Blackman-Harris variant (__IBH__)
 
 
+@pytest.mark.parametrize("prof", PROFILES)
 @pytest.mark.parametrize("cfg", [
     dict(nprocs=(2, 2, 2), L=(32, 32, 32), h=10),
     dict(nprocs=(3, 1, 2), L=(40, 24, 30), h=9),
     dict(nprocs=(1, 1, 1), L=(24, 20, 22), h=10),
 ])
-def test_step_against_dense_global(cfg):
+def test_step_against_dense_global(cfg, prof):
     """One Update from a random state equals the dense global restatement
     (windowed D2 matrices from numpy's FFT), to 1e-12."""
     p, V = W.config_tiny(**cfg)
+    p.profile = prof
     o = O.Oracle(p, V)
     rng = np.random.default_rng(11)
     a = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
@@ -42,11 +43,14 @@ def test_step_against_dense_global(cfg):
         assert o.time_index() == ntau + 1
 
 
-def test_initial_field_closed_form():
+@pytest.mark.parametrize("profile", PROFILES)
+def test_initial_field_closed_form(profile):
     """psi(0) = Xi e^{i k.(r - r0)}, psi(dtau) = psi(0) e^{-i dtau} on the TF
     box, zero outside; m_nTau = 1."""
     p, _ = W.config_tiny(theta=0.7, phi=2.2, well=None)
+    p.profile = profile
     o = O.Oracle(p, None)
+    _xi_np = lambda s: BF.xi(s, profile)
     assert o.time_index() == 1
     N = o.N
     n0t1 = p.n_ksi - 1
@@ -66,13 +70,16 @@ def test_initial_field_closed_form():
     assert np.allclose(o.get_psi(0), psi0 * np.exp(-1j * p.dtau), rtol=0, atol=1e-14)
 
 
+@pytest.mark.parametrize("profile", PROFILES)
 @pytest.mark.parametrize("theta,phi", [(0.4, 0.9), (2.5, -2.0), (0.0, 0.0)])
-def test_tfsf_source_is_commutator(theta, phi):
+def test_tfsf_source_is_commutator(theta, phi, profile):
     """Sum of the six Inc slabs = 2 dtau (lap Xi + 2 i khat.grad Xi) e^{i r}
     with Xi the two-sided closed-form profile; derivatives of Xi taken by
     5-point finite differences of xi at continuous arguments."""
     p, _ = W.config_tiny(theta=theta, phi=phi, well=None, nprocs=(2, 1, 2), L=(36, 40, 36), h=10)
+    p.profile = profile
     o = O.Oracle(p, None)
+    _xi_np = lambda s: BF.xi(s, profile)
     S = o.get_source()
     N = o.N
     n0t1 = p.n_ksi - 1
@@ -105,4 +112,12 @@ def test_tfsf_source_is_commutator(theta, phi):
     # the inner region (strictly between transition layers) has no source
     scale = np.abs(ref).max()
     assert scale > 0
-    assert np.abs(S - ref).max() / scale < 2e-6
+    keep = np.ones(S.shape, bool)
+    if profile == 1:
+        # the integrated Blackman-Harris xi has xi'(0) = xi'(1) = A0+A1+A2+A3 = 1e-4,
+        # which the clamp at the layer ends sets to 0: a kink that a finite-difference
+        # stencil across the ends cannot see, so those planes are left out
+        for a in range(3):
+            ends = np.isin(np.arange(N[a]), [o.info[k][a] for k in ("ksi1a", "ksi1b", "ksi2a", "ksi2b")])
+            keep &= ~e(ends, a)
+    assert np.abs(S - ref)[keep].max() / scale < 2e-6
