@@ -268,6 +268,15 @@ int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]
  * that axis (FFT needs L = 2^a 3^b 5^c 7^d <= 512).  No reference
  * counterpart: the reference always uses MKL DFTI (PSTD_QScat.cpp:496-526). */
 int pstd_set_method(pstd_solver *s, int axis, int method);
+
+/* Cache policy of the passes' psi^{n-1} read-modify-write: 1 reads and
+ * writes it with non-temporal hints, 0 with plain ones, -1 (the default)
+ * picks 1 when psi and psi^{n-1} together exceed MI355X's 256 MiB
+ * last-level cache, where psi^{n-1} only streams through it, and 0 when they
+ * fit, where the cache carries it from one pass to the next.  Results are
+ * identical either way; only the speed differs.  (No counterpart in the
+ * reference.) */
+int pstd_set_cache_streaming(pstd_solver *s, int mode);
 int pstd_get_method(const pstd_solver *s, int axis);
 
 /* Autotuner: for each axis, time every available method's pass (reps

@@ -98,6 +98,37 @@ __device__ __forceinline__ void stv(GLOBAL_AS float2 *p, const float2 &v) {
     *(GLOBAL_AS nat_f2 *)p = nat_f2{v.x, v.y};
 }
 
+// psi0 accesses of the pass epilogues: a plain or a non-temporal load/store,
+// chosen at compile time (the register-tiled kernels come in both forms and
+// the launcher picks one; a run-time branch between the two lets the compiler
+// merge them and drop the hint).  When psi and psi0 together exceed the
+// last-level cache, psi0 only streams through it -- each pass reads and
+// writes it once -- and evicts the psi lines whose halo re-reads the
+// neighbouring segments' blocks need; when they fit, the cache carries psi0
+// from one pass to the next and the plain path is faster.
+template <bool NT> __device__ __forceinline__ double2 ld_psi0(const GLOBAL_AS double2 *p) {
+    if (!NT) return ldv(p);
+    const nat_d2 v = __builtin_nontemporal_load((const GLOBAL_AS nat_d2 *)p);
+    return make_double2(v.x, v.y);
+}
+template <bool NT> __device__ __forceinline__ float2 ld_psi0(const GLOBAL_AS float2 *p) {
+    if (!NT) return ldv(p);
+    const nat_f2 v = __builtin_nontemporal_load((const GLOBAL_AS nat_f2 *)p);
+    return make_float2(v.x, v.y);
+}
+template <bool NT> __device__ __forceinline__ void st_psi0(GLOBAL_AS double2 *p, const double2 &v) {
+    if (NT) __builtin_nontemporal_store(nat_d2{v.x, v.y}, (GLOBAL_AS nat_d2 *)p);
+    else stv(p, v);
+}
+template <bool NT> __device__ __forceinline__ void st_psi0(GLOBAL_AS float2 *p, const float2 &v) {
+    if (NT) __builtin_nontemporal_store(nat_f2{v.x, v.y}, (GLOBAL_AS nat_f2 *)p);
+    else stv(p, v);
+}
+template <bool NT> __device__ __forceinline__ double2 ld_psi0(const double2 *p) { return ld_psi0<NT>(glb(p)); }
+template <bool NT> __device__ __forceinline__ float2 ld_psi0(const float2 *p) { return ld_psi0<NT>(glb(p)); }
+template <bool NT> __device__ __forceinline__ void st_psi0(double2 *p, const double2 &v) { st_psi0<NT>(glb(p), v); }
+template <bool NT> __device__ __forceinline__ void st_psi0(float2 *p, const float2 &v) { st_psi0<NT>(glb(p), v); }
+
 __host__ __device__ constexpr size_t align16(size_t n) { return (n + 15) & ~size_t(15); }
 
 template <typename T> size_t plane_lds_bytes(int MB, int KB) {
@@ -126,7 +157,7 @@ template <typename C2> __device__ __forceinline__ C2 czero() {
 }
 
 // psi0[core row m][col] += -i U for the NHB 16-point blocks of a x/y tile
-template <typename T, int NHB, typename P>
+template <typename T, int NHB, bool NT, typename P>
 __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, int o, long c0,
                                                long core1, int ncore, int wave, int kq, int cq,
                                                const typename VT<T>::acc &ar0, const typename VT<T>::acc &ai0,
@@ -138,16 +169,16 @@ __device__ __forceinline__ void plane_epilogue(const PlaneArgs<T> &a, P psi0, in
         if (m >= ncore) continue;
         auto dst = psi0 + (long)o * a.outer_stride + (core1 + m) * a.row_stride + c0;
         if (c0 + cq < a.ncols) {
-            C2 v = ldv(dst + cq);
+            C2 v = ld_psi0<NT>(dst + cq);
             v.x += ai0[q];
             v.y -= ar0[q];
-            stv(dst + cq, v);
+            st_psi0<NT>(dst + cq, v);
         }
         if (NHB == 2 && c0 + 16 + cq < a.ncols) {
-            C2 v = ldv(dst + 16 + cq);
+            C2 v = ld_psi0<NT>(dst + 16 + cq);
             v.x += ai1[q];
             v.y -= ar1[q];
-            stv(dst + 16 + cq, v);
+            st_psi0<NT>(dst + 16 + cq, v);
         }
     }
 }
@@ -247,12 +278,12 @@ __device__ __forceinline__ ZSrc z_src(const LineArgs<T> &a, int z) {
 // One output point of the fused z pass: psi0 at dst, U = alpha D2 (W psi)
 // at the point (alpha folded), pn = psi^n at the point, v = V~ at the point
 // (0 outside the potential box; the ramp factor is applied here).
-template <typename T, typename P>
+template <typename T, bool NT, typename P>
 __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &clk, const LineInfo &li,
                                         const ZInfo &zi, const ZSrc &zs, P dst, T ur, T ui,
                                         typename VT<T>::c2 pn, T v) {
     typedef typename VT<T>::c2 C2;
-    C2 acc = ldv(dst);
+    C2 acc = ld_psi0<NT>(dst);
     acc.x += ui;
     acc.y -= ur;
     v *= clk.vfac;
@@ -272,7 +303,7 @@ __device__ __forceinline__ void z_point(const LineArgs<T> &a, const Clock<T> &cl
     const double gg = li.gxy * zi.gz;
     acc.x = (T)(acc.x * gg);
     acc.y = (T)(acc.y * gg);
-    stv(dst, acc);
+    st_psi0<NT>(dst, acc);
 }
 
 // V~ at the point of line li and z (a global load; the register variant
@@ -283,7 +314,7 @@ __device__ __forceinline__ T z_potential(const LineArgs<T> &a, const LineInfo &l
 }
 
 // PV: vq[q] holds V~ of result element q (NHB = 1), prefetched by the caller
-template <typename T, int NHB, bool PV>
+template <typename T, int NHB, bool PV, bool NT>
 __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<T> &clk, const typename VT<T>::c2 *Ls,
                                               const LineInfo *LI,
                                               int LS, const ZInfo &zi, const ZSrc &zs, int z, int m, int i1, int kq,
@@ -300,7 +331,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
             const LineInfo &li = LI[line];
             if (li.base < 0) continue;
             const T v = PV ? vq[q] : z_potential<T>(a, li, zi);
-            z_point<T>(a, clk, li, zi, zs, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
+            z_point<T, NT>(a, clk, li, zi, zs, psi0 + li.base + z, hb ? ar1[q] : ar0[q], hb ? ai1[q] : ai0[q],
                        Ls[line * LS + i1 + m], v);
         }
     }
@@ -311,7 +342,7 @@ __device__ __forceinline__ void line_epilogue(const LineArgs<T> &a, const Clock<
 // strided set of column tiles; wave w owns 16-row block w of the core and
 // keeps its operator fragments in registers for the life of the block.
 // ---------------------------------------------------------------------------
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
@@ -382,15 +413,15 @@ __device__ __forceinline__ void plane_reg_body(const PlaneArgs<T> &a, int r, int
                     }
                 }
             }
-            plane_epilogue<T, NHB>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
+            plane_epilogue<T, NHB, NT>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
         }
     }
 }
 
 
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
-    plane_reg_body<T, KBC>(a, blockIdx.y, blockIdx.x, gridDim.x);
+    plane_reg_body<T, KBC, NT>(a, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -399,7 +430,7 @@ __global__ void __launch_bounds__(512) k_lap_plane_reg(PlaneArgs<T> a) {
 // operand, the operator the B operand) so that each lane ends with both
 // components of one output point and 16 lanes cover 16 consecutive z.
 // ---------------------------------------------------------------------------
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<T> &clk, int r, int bx, int gx) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
@@ -522,15 +553,16 @@ __device__ __forceinline__ void line_reg_body(const LineArgs<T> &a, const Clock<
                 }
             }
         }
-        if (lane_ok) line_epilogue<T, NHB, true>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1, vq);
+        if (lane_ok)
+            line_epilogue<T, NHB, true, NT>(a, clk, Ls, LI, LS, zi, ZT[m], z, m, i1, kq, ar0, ai0, ar1, ai1, vq);
         if (t + gx < ntiles) fetch_v(t + gx);
     }
 }
 
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __global__ void __launch_bounds__(512) k_lap_line_reg(LineArgs<T> a) {
     const Clock<T> clk = {a.sin_t, a.cos_t, a.vfac};
-    line_reg_body<T, KBC>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
+    line_reg_body<T, KBC, NT>(a, clk, blockIdx.y, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -553,7 +585,7 @@ __device__ __forceinline__ void seg_slice(SegTable &d, const SegTable &s, int r)
     d.opoff[0] = s.opoff[r];
 }
 
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restrict__ list,
                                                      const int2 *__restrict__ map) {
     const int2 br = map[blockIdx.y];
@@ -570,10 +602,10 @@ __global__ void __launch_bounds__(512) k_batch_plane(const PlaneArgs<T> *__restr
     a.L = g.L;
     a.KB = g.KB;
     seg_slice<T>(a.seg, g.seg, br.y);
-    plane_reg_body<T, KBC>(a, 0, blockIdx.x, gridDim.x);
+    plane_reg_body<T, KBC, NT>(a, 0, blockIdx.x, gridDim.x);
 }
 
-template <typename T, int KBC>
+template <typename T, int KBC, bool NT>
 __global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restrict__ list, const int2 *__restrict__ map,
                                                     const double *__restrict__ clock, const int *__restrict__ step,
                                                     int nprob) {
@@ -608,7 +640,7 @@ __global__ void __launch_bounds__(512) k_batch_line(const LineArgs<T> *__restric
         a.k2b[d] = g.k2b[d];
         a.g[d] = g.g[d];
     }
-    line_reg_body<T, KBC>(a, clk, 0, blockIdx.x, gridDim.x);
+    line_reg_body<T, KBC, NT>(a, clk, 0, blockIdx.x, gridDim.x);
 }
 
 // Time-independent per-line terms (line_info at unit time factor, so that
@@ -678,7 +710,7 @@ __global__ void __launch_bounds__(1024) k_lap_plane_lds(PlaneArgs<T> a) {
                 ar1 = mfma16(af, b1.x, ar1);
                 ai1 = mfma16(af, b1.y, ai1);
             }
-            plane_epilogue<T, 2>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
+            plane_epilogue<T, 2, false>(a, psi0, o, c0, core1, ncore, wave, kq, cq, ar0, ai0, ar1, ai1);
         }
     }
 }
@@ -736,7 +768,8 @@ __global__ void __launch_bounds__(1024) k_lap_line_lds(LineArgs<T> a) {
             ai1 = mfma16(p1.y, af, ai1);
         }
         const T vnone[4] = {T(0), T(0), T(0), T(0)};
-        if (lane_ok) line_epilogue<T, 2, false>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1, vnone);
+        if (lane_ok)
+            line_epilogue<T, 2, false, false>(a, clk, Ls, LI, LS, zi, zs, z, m, i1, kq, ar0, ai0, ar1, ai1, vnone);
     }
 }
 
@@ -1042,7 +1075,7 @@ __global__ void __launch_bounds__(FNTHR) k_fft_line(LineArgs<T> a) {
             const int z = core1 + m;
             const ZInfo zi = z_info<T>(a, z);
             const C2 u = S[(i1 + m) * LDW + c];
-            z_point<T>(a, clk, li, zi, z_src<T>(a, z), psi0 + li.base + z, u.x, u.y, psi[li.base + z],
+            z_point<T, false>(a, clk, li, zi, z_src<T>(a, z), psi0 + li.base + z, u.x, u.y, psi[li.base + z],
                        z_potential<T>(a, li, zi));
         }
     }
@@ -1060,39 +1093,19 @@ template <typename T> int fft_occupancy(int L, bool line) {
 }
 
 // ---------------------------------------------------------------------------
-// Register-variant instantiation for a k-block count.
-template <typename T> const void *plane_reg_kernel(int KB) {
-    switch (KB) {
-    case 16: return (const void *)k_lap_plane_reg<T, 16>;
-    case 20: return (const void *)k_lap_plane_reg<T, 20>;
-    case 24: return (const void *)k_lap_plane_reg<T, 24>;
-    default: return (const void *)k_lap_plane_reg<T, 0>;
+// Register-variant instantiation for a k-block count and psi0 cache policy.
+#define PSTD_KB_SWITCH(K, NT)                                                                   \
+    switch (KB) {                                                                              \
+    case 16: return NT ? (const void *)K<T, 16, true> : (const void *)K<T, 16, false>;         \
+    case 20: return NT ? (const void *)K<T, 20, true> : (const void *)K<T, 20, false>;         \
+    case 24: return NT ? (const void *)K<T, 24, true> : (const void *)K<T, 24, false>;         \
+    default: return NT ? (const void *)K<T, 0, true> : (const void *)K<T, 0, false>;           \
     }
-}
-template <typename T> const void *line_reg_kernel(int KB) {
-    switch (KB) {
-    case 16: return (const void *)k_lap_line_reg<T, 16>;
-    case 20: return (const void *)k_lap_line_reg<T, 20>;
-    case 24: return (const void *)k_lap_line_reg<T, 24>;
-    default: return (const void *)k_lap_line_reg<T, 0>;
-    }
-}
-template <typename T> const void *batch_plane_kernel(int KB) {
-    switch (KB) {
-    case 16: return (const void *)k_batch_plane<T, 16>;
-    case 20: return (const void *)k_batch_plane<T, 20>;
-    case 24: return (const void *)k_batch_plane<T, 24>;
-    default: return (const void *)k_batch_plane<T, 0>;
-    }
-}
-template <typename T> const void *batch_line_kernel(int KB) {
-    switch (KB) {
-    case 16: return (const void *)k_batch_line<T, 16>;
-    case 20: return (const void *)k_batch_line<T, 20>;
-    case 24: return (const void *)k_batch_line<T, 24>;
-    default: return (const void *)k_batch_line<T, 0>;
-    }
-}
+template <typename T> const void *plane_reg_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_lap_plane_reg, nt) }
+template <typename T> const void *line_reg_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_lap_line_reg, nt) }
+template <typename T> const void *batch_plane_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_batch_plane, nt) }
+template <typename T> const void *batch_line_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_batch_line, nt) }
+#undef PSTD_KB_SWITCH
 
 template <typename T>
 hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
@@ -1103,7 +1116,8 @@ hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
         PlaneArgs<T> arg = a;
         void *args[] = {&arg};
-        return hipLaunchKernel(plane_reg_kernel<T>(a.KB), grid, block, args, plane_reg_lds_bytes<T>(a.KB), s);
+        return hipLaunchKernel(plane_reg_kernel<T>(a.KB, a.nt != 0), grid, block, args, plane_reg_lds_bytes<T>(a.KB),
+                               s);
     } else {
         dim3 block(64 * a.MB);
         hipLaunchKernelGGL(k_lap_plane_lds<T>, grid, block, plane_lds_bytes<T>(a.MB, a.KB), s, a);
@@ -1119,7 +1133,7 @@ hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s) {
         dim3 block(64 * (a.MB > 4 ? a.MB : 4));
         LineArgs<T> arg = a;
         void *args[] = {&arg};
-        return hipLaunchKernel(line_reg_kernel<T>(a.KB), grid, block, args, line_reg_lds_bytes<T>(a.KB), s);
+        return hipLaunchKernel(line_reg_kernel<T>(a.KB, a.nt != 0), grid, block, args, line_reg_lds_bytes<T>(a.KB), s);
     } else {
         dim3 block(64 * a.MB);
         hipLaunchKernelGGL(k_lap_line_lds<T>, grid, block, line_lds_bytes<T>(a.MB, a.KB), s, a);
@@ -1135,16 +1149,16 @@ hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStre
 
 template <typename T>
 hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
-                              bool kb_uniform, hipStream_t s) {
+                              bool kb_uniform, bool nt, hipStream_t s) {
     void *args[] = {&list, &map};
-    return hipLaunchKernel(batch_plane_kernel<T>(kb_uniform ? KB : 0), dim3(grid_x, nmap), dim3(block), args,
+    return hipLaunchKernel(batch_plane_kernel<T>(kb_uniform ? KB : 0, nt), dim3(grid_x, nmap), dim3(block), args,
                            plane_reg_lds_bytes<T>(KB), s);
 }
 template <typename T>
 hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
-                             int nprob, int grid_x, int block, int KB, bool kb_uniform, hipStream_t s) {
+                             int nprob, int grid_x, int block, int KB, bool kb_uniform, bool nt, hipStream_t s) {
     void *args[] = {&list, &map, &clock, &step, &nprob};
-    return hipLaunchKernel(batch_line_kernel<T>(kb_uniform ? KB : 0), dim3(grid_x, nmap), dim3(block), args,
+    return hipLaunchKernel(batch_line_kernel<T>(kb_uniform ? KB : 0, nt), dim3(grid_x, nmap), dim3(block), args,
                            line_reg_lds_bytes<T>(KB), s);
 }
 hipError_t launch_tick(int *step, hipStream_t s) {
@@ -1214,9 +1228,10 @@ template <typename T> int pass_occupancy(int MB, int KB, bool line) {
     if (reg_variant_ok(MB, KB)) {
         const int block = 64 * (MB > 4 ? MB : 4);
         if (line)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, line_reg_kernel<T>(KB), block, line_reg_lds_bytes<T>(KB));
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, line_reg_kernel<T>(KB, false), block,
+                                                             line_reg_lds_bytes<T>(KB));
         else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, plane_reg_kernel<T>(KB), block,
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, plane_reg_kernel<T>(KB, false), block,
                                                              plane_reg_lds_bytes<T>(KB));
     } else {
         const int block = 64 * MB;
@@ -1244,16 +1259,17 @@ hipError_t configure_kernels(int max_lds) {
                                    (const void *)k_lap_line_lds<double>,  (const void *)k_lap_line_lds<float>,
                                    (const void *)k_fft_plane<double>,     (const void *)k_fft_plane<float>,
                                    (const void *)k_fft_line<double>,      (const void *)k_fft_line<float>};
-    for (int kb : {0, 16, 20, 24}) {
-        k.push_back(plane_reg_kernel<double>(kb));
-        k.push_back(plane_reg_kernel<float>(kb));
-        k.push_back(line_reg_kernel<double>(kb));
-        k.push_back(line_reg_kernel<float>(kb));
-        k.push_back(batch_plane_kernel<double>(kb));
-        k.push_back(batch_plane_kernel<float>(kb));
-        k.push_back(batch_line_kernel<double>(kb));
-        k.push_back(batch_line_kernel<float>(kb));
-    }
+    for (int kb : {0, 16, 20, 24})
+        for (bool nt : {false, true}) {
+            k.push_back(plane_reg_kernel<double>(kb, nt));
+            k.push_back(plane_reg_kernel<float>(kb, nt));
+            k.push_back(line_reg_kernel<double>(kb, nt));
+            k.push_back(line_reg_kernel<float>(kb, nt));
+            k.push_back(batch_plane_kernel<double>(kb, nt));
+            k.push_back(batch_plane_kernel<float>(kb, nt));
+            k.push_back(batch_line_kernel<double>(kb, nt));
+            k.push_back(batch_line_kernel<float>(kb, nt));
+        }
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return e;
@@ -1278,12 +1294,12 @@ template size_t fft_lds_bytes<float>(int);
 template int fft_occupancy<double>(int, bool);
 template int fft_occupancy<float>(int, bool);
 template hipError_t launch_batch_plane<double>(const PlaneArgs<double> *, const int2 *, int, int, int, int, bool,
-                                               hipStream_t);
-template hipError_t launch_batch_plane<float>(const PlaneArgs<float> *, const int2 *, int, int, int, int, bool,
+                                               bool, hipStream_t);
+template hipError_t launch_batch_plane<float>(const PlaneArgs<float> *, const int2 *, int, int, int, int, bool, bool,
                                               hipStream_t);
 template hipError_t launch_batch_line<double>(const LineArgs<double> *, const int2 *, int, const double *, const int *,
-                                              int, int, int, int, bool, hipStream_t);
+                                              int, int, int, int, bool, bool, hipStream_t);
 template hipError_t launch_batch_line<float>(const LineArgs<float> *, const int2 *, int, const double *, const int *,
-                                             int, int, int, int, bool, hipStream_t);
+                                             int, int, int, int, bool, bool, hipStream_t);
 
 }  // namespace pstd

@@ -49,6 +49,7 @@ struct PlaneArgs {
     SegTable seg;
     int method;          // PSTD_METHOD_GEMM or PSTD_METHOD_FFT
     int fcol_tiles;      // column tiles of the FFT variant
+    int nt;              // psi0 with non-temporal hints: the launcher's kernel choice
     FftPlan<T> fft;
 };
 
@@ -82,6 +83,7 @@ struct LineArgs {
     const void *linfo;
     int method;          // PSTD_METHOD_GEMM or PSTD_METHOD_FFT
     int fline_tiles;     // line tiles of the FFT variant
+    int nt;              // psi0 with non-temporal hints: the launcher's kernel choice
     FftPlan<T> fft;
 };
 
@@ -112,10 +114,10 @@ bool batch_variant_ok(int MB, int KB);
 // problem has that KB, so the compile-time instantiation may be used.
 template <typename T>
 hipError_t launch_batch_plane(const PlaneArgs<T> *list, const int2 *map, int nmap, int grid_x, int block, int KB,
-                              bool kb_uniform, hipStream_t s);
+                              bool kb_uniform, bool nt, hipStream_t s);
 template <typename T>
 hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
-                             int nprob, int grid_x, int block, int KB, bool kb_uniform, hipStream_t s);
+                             int nprob, int grid_x, int block, int KB, bool kb_uniform, bool nt, hipStream_t s);
 hipError_t launch_tick(int *step, hipStream_t s);
 constexpr size_t LINE_RECORD_BYTES = 64;
 // psi part of the six faces' accumulators: face f's plane_n points start at

@@ -84,6 +84,7 @@ struct pstd_solver {
     long plane_off[6], plane_n[6];
     long nTau = 1, nSur = 0;
     int file_prec = PSTD_FP64;    // surface-file precision (header reals and body)
+    int nt_mode = -1;             // psi0 streaming: -1 automatic, 0 off, 1 on
     std::vector<void *> allocs;
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -323,6 +324,16 @@ FftPlan<T> fft_plan(pstd_solver *s, int a) {
     return f;
 }
 
+// MI355X's last-level cache (Infinity Cache).  When psi and psi0 together
+// exceed it, the passes read and write psi0 with non-temporal hints.
+constexpr double LLC_BYTES = 256.0 * 1024 * 1024;
+
+int stream_psi0(const pstd_solver *s, double working_set) {
+    if (s->nt_mode >= 0) return s->nt_mode;
+    return working_set > LLC_BYTES ? 1 : 0;
+}
+double level_pair_bytes(const pstd_solver *s) { return 2.0 * (double)s->npts() * 2.0 * (double)s->esz; }
+
 // Arguments of the x (axis 0) and y (axis 1) passes from psi into psi0.
 template <typename T>
 PlaneArgs<T> plane_args(pstd_solver *s, int axis, const T *psi, T *psi0) {
@@ -340,6 +351,7 @@ PlaneArgs<T> plane_args(pstd_solver *s, int axis, const T *psi, T *psi0) {
     a.fcol_tiles = (a.ncols + FFT_NC - 1) / FFT_NC;
     a.L = (int)g.ax[axis].L; a.KB = s->lay[axis].KB; a.MB = s->lay[axis].MB; a.seg = s->seg[axis];
     a.method = s->method[axis];
+    a.nt = stream_psi0(s, level_pair_bytes(s));
     a.fft = fft_plan<T>(s, axis);
     return a;
 }
@@ -358,6 +370,7 @@ LineArgs<T> line_args(pstd_solver *s, const T *psi, T *psi0, long ntau) {
     a.fline_tiles = (int)((a.nlines + FFT_NC - 1) / FFT_NC);
     a.seg = s->seg[2];
     a.method = s->method[2];
+    a.nt = stream_psi0(s, level_pair_bytes(s));
     a.fft = fft_plan<T>(s, 2);
     a.vbox = reinterpret_cast<const T *>(s->vbox);
     for (int d = 0; d < 3; ++d) {
@@ -783,6 +796,12 @@ int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long 
     return 0;
 }
 
+int pstd_set_cache_streaming(pstd_solver *s, int mode) {
+    if (!s || mode < -1 || mode > 1) return fail(PSTD_EINVAL, "streaming mode must be -1, 0 or 1");
+    s->nt_mode = mode;
+    return 0;
+}
+
 int pstd_set_file_precision(pstd_solver *s, int precision) {
     if (!s || (precision != PSTD_FP64 && precision != PSTD_FP32))
         return fail(PSTD_EINVAL, "file precision must be 8 (FP64) or 4 (FP32)");
@@ -1057,6 +1076,7 @@ struct pstd_batch {
     size_t esz = 8;
     int KB = 0, block = 256;
     bool kb_uniform[3] = {true, true, true};   // every problem has KB on that axis
+    bool nt = false;              // psi0 streamed with non-temporal hints
     void *lists[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [parity][pass]
     int2 *maps[3] = {nullptr, nullptr, nullptr};
     int nmap[3] = {0, 0, 0};
@@ -1134,6 +1154,11 @@ int batch_build(pstd_batch *b) {
                 max_tiles[2] = std::max(max_tiles[2], (long)lz.back().line_tiles);
             }
         }
+        // the batch streams all members through the cache together
+        double total = 0.0;
+        for (int i = 0; i < nprob; ++i) total += level_pair_bytes(b->sv[i]);
+        b->nt = stream_psi0(b->sv[0], total) != 0;
+        for (int i = 0; i < nprob; ++i) lx[i].nt = ly[i].nt = lz[i].nt = b->nt ? 1 : 0;
         HIPCHK(hipMalloc(&b->lists[p][0], lx.size() * sizeof(PlaneArgs<T>)), "batch args");
         HIPCHK(hipMalloc(&b->lists[p][1], ly.size() * sizeof(PlaneArgs<T>)), "batch args");
         HIPCHK(hipMalloc(&b->lists[p][2], lz.size() * sizeof(LineArgs<T>)), "batch args");
@@ -1163,14 +1188,16 @@ int batch_step_launch(pstd_batch *b, int parity) {
     // batch's KB on that axis
     const int KB = b->KB;
     HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][0]), b->maps[0], b->nmap[0],
-                                 b->gx[0], b->block, KB, b->kb_uniform[0] && b->sv[0]->lay[0].KB == KB, b->stream),
+                                 b->gx[0], b->block, KB, b->kb_uniform[0] && b->sv[0]->lay[0].KB == KB, b->nt,
+                                 b->stream),
            "batch x pass");
     HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][1]), b->maps[1], b->nmap[1],
-                                 b->gx[1], b->block, KB, b->kb_uniform[1] && b->sv[0]->lay[1].KB == KB, b->stream),
+                                 b->gx[1], b->block, KB, b->kb_uniform[1] && b->sv[0]->lay[1].KB == KB, b->nt,
+                                 b->stream),
            "batch y pass");
     HIPCHK(launch_batch_line<T>(reinterpret_cast<const LineArgs<T> *>(b->lists[parity][2]), b->maps[2], b->nmap[2],
                                 b->clock, b->step, nprob, b->gx[2], b->block, KB,
-                                b->kb_uniform[2] && b->sv[0]->lay[2].KB == KB, b->stream),
+                                b->kb_uniform[2] && b->sv[0]->lay[2].KB == KB, b->nt, b->stream),
            "batch z pass");
     HIPCHK(launch_tick(b->step, b->stream), "batch tick");
     return 0;

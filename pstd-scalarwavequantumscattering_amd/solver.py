@@ -26,7 +26,7 @@ EXPORTS = [
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
-    "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
+    "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_set_cache_streaming", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
 ]
@@ -95,6 +95,8 @@ def library():
         L.pstd_far_field_file.argtypes = [C.c_char_p, dp, l, dp, dp]
         L.pstd_incident_amplitude.restype = i
         L.pstd_incident_amplitude.argtypes = [C.POINTER(PstdParams), dp]
+        L.pstd_set_cache_streaming.restype = i
+        L.pstd_set_cache_streaming.argtypes = [vp, i]
         L.pstd_set_file_precision.restype = i
         L.pstd_set_file_precision.argtypes = [vp, i]
         L.pstd_save_g.restype = i
@@ -250,6 +252,10 @@ class PSTDQScat:
 
     def reset_surface_terms(self):
         _check(library().pstd_reset_surface_terms(self.h))
+
+    def set_cache_streaming(self, mode):
+        """psi^{n-1} accesses with non-temporal hints (1), plain (0) or by working set (-1)."""
+        _check(library().pstd_set_cache_streaming(self.h, int(mode)))
 
     def set_file_precision(self, precision):
         """Surface files as FP64 (8) or, like the reference's FP32 build, FP32 (4)."""

@@ -222,3 +222,26 @@ def test_fp32_surface_files_are_float(tmp_path):
     with pytest.raises(pstd_amd.PSTDError):
         g.set_file_precision(2)
     g.close()
+
+
+@pytest.mark.parametrize("precision", [pstd_amd.PSTD_FP64, pstd_amd.PSTD_FP32])
+def test_cache_streaming_is_bit_identical(precision):
+    """The non-temporal psi^{n-1} path (taken automatically above 256 MiB)
+    computes exactly what the plain path does, on a tiny grid where it is
+    forced, through all three passes and the surface capture."""
+    p, V = W.config_tiny(precision=precision, **CASES[0])
+    print(f"case cache streaming prec={precision}", flush=True)
+    runs = []
+    for mode in (0, 1):
+        g = pstd_amd.PSTDQScat(p, V)
+        g.set_cache_streaming(mode)
+        g.Update(15)
+        g.update_accumulate(5)
+        runs.append((g.get_psi(0), g.get_psi(1), g.get_surfaces(0)))
+        g.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    g = pstd_amd.PSTDQScat(p, V)
+    with pytest.raises(pstd_amd.PSTDError):
+        g.set_cache_streaming(2)
+    g.close()
