@@ -1,0 +1,72 @@
+"""GPU parity at the benchmark sizes, against the CPU oracle.
+
+grf224: the headline workload itself (224^3, 4x4x4 subdomains of L = 80,
+h = 16, Gaussian random potential, FP64), where the kernels run the KB = 20
+register instantiations and, psi and psi^(n-1) together exceeding the
+last-level cache, the streamed psi^(n-1) variant: a few steps from the
+physical start, then a surface window from a random state, to 1e-10.
+wells320: the 320^3 FP32 workload (6x6x6 subdomains, FP64 phase) against
+the FP64 oracle, to the FP32 bar of the small cases (2e-5).  Few steps keep the oracle's share of the test near
+ten seconds; the tiny and 96^3 cases in test_gpu_parity.py cover long
+trajectories."""
+import numpy as np
+import pytest
+
+import pstd_amd
+from pstd_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _random_state_window(o, g, tol):
+    """From the same random psi^n, psi^(n-1) on the whole grid: two steps
+    and a three-step surface window.  (From the physical start the scattered
+    wave needs thousands of steps to reach the virtual surfaces, which until
+    then hold only rounding-level leakage.)"""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    b = rng.standard_normal(o.N) + 1j * rng.standard_normal(o.N)
+    for s in (o, g):
+        s.set_psi(0, a)
+        s.set_psi(1, b)
+        s.set_time_index(2000)        # past the potential ramp
+    del a, b
+    o.reset_surface_terms()
+    g.reset_surface_terms()
+    o.update(2)
+    g.Update(2)
+    assert rel(g.get_psi(0), o.get_psi(0)) < tol
+    o.update_accumulate(3)
+    g.update_accumulate(3)
+    for deriv in (0, 1):
+        assert rel(g.get_surfaces(deriv), o.get_surfaces(deriv)) < tol
+
+
+def test_grf224_fp64_against_oracle():
+    p, V = W.config_grf224(precision=pstd_amd.PSTD_FP64)
+    print("case grf224 FP64 224^3 4x4x4: 6 steps, then random state 2 + 3 accumulation steps", flush=True)
+    o = O.Oracle(p, V)
+    g = pstd_amd.PSTDQScat(p, V)
+    o.update(6)
+    g.Update(6)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 1e-10
+    assert rel(g.get_psi(1), o.get_psi(1)) < 1e-10
+    _random_state_window(o, g, 1e-10)
+    g.close()
+
+
+def test_wells320_fp32_against_fp64_oracle():
+    p, V = W.config_wells320(precision=pstd_amd.PSTD_FP32)
+    print("case wells320 FP32 320^3 6x6x6: 3 steps, then random state 2 + 3 accumulation steps", flush=True)
+    o = O.Oracle(p, V)
+    g = pstd_amd.PSTDQScat(p, V)
+    o.update(3)
+    g.Update(3)
+    assert rel(g.get_psi(0), o.get_psi(0)) < 2e-5
+    _random_state_window(o, g, 2e-5)
+    g.close()
