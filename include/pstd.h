@@ -211,6 +211,33 @@ int pstd_reset_surface_terms(pstd_solver *s);
 int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long *periods_out, double *history,
                              int *converged_out);
 
+/* Steady state with reduced-rank extrapolation (RRE; A. Sidi, Vector
+ * Extrapolation Methods with Applications, SIAM 2017) of the one-period
+ * window sequence.  No counterpart in the reference; for callers who want
+ * the steady state in fewer periods than the windows themselves converge.
+ * Windows run back to back as in pstd_run_to_steady_state; after each one
+ * the last min(k, rank + 2) windows give an estimate of the limit (weights
+ * from the surface psi of the six faces), and the run stops when two
+ * successive estimates differ by at most tol (relative L2 over the surface
+ * psi) or after max_periods windows.  history[k] (may be NULL, max_periods
+ * entries) receives that change (the plain window change while fewer than
+ * three windows exist); raw_history[k] (may be NULL) the plain change
+ * between windows k-1 and k.  On return the accumulators hold the estimate
+ * for psi and both derivative kinds, normalised as one window, for
+ * pstd_get_surfaces, pstd_save_virtual_surfaces and pstd_far_field.  The
+ * device runs window k+1 while the host extrapolates from window k, so a
+ * run that stops before max_periods has stepped one window further than the
+ * windows it used (time index included).  Host memory: rank + 2 copies of the
+ * accumulators.  rank 1..16. */
+#define PSTD_RRE_MAX_RANK 16
+int pstd_run_to_steady_state_rre(pstd_solver *s, double tol, long max_periods, int rank, long *periods_out,
+                                 double *history, double *raw_history, int *converged_out);
+
+/* The RRE weights of k vectors (k = 3 .. PSTD_RRE_MAX_RANK + 2) of n complex
+ * values each, x laid out [k][n][re, im]: gamma_out receives k - 1 complex
+ * weights (summing to 1) of the estimate sum_i gamma_i x_i.  Host only. */
+int pstd_rre_weights(const double *x, long n, int k, double *gamma_out);
+
 /* ---- diagnostics ------------------------------------------------------ */
 
 /* CPSTD_QScat::Save_g (PSTD_QScat.cpp:2191-2253): for every subdomain (rank

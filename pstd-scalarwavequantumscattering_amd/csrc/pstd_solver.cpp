@@ -499,6 +499,90 @@ int fetch_acc(pstd_solver *s, std::vector<double2> &acc) {
     return 0;
 }
 
+// Reduced-rank extrapolation (RRE; A. Sidi, Vector Extrapolation Methods with
+// Applications, SIAM 2017, ch. 1-2).  For vectors x_0 .. x_{k-1} with
+// differences u_i = x_{i+1} - x_i, the weights gamma_0 .. gamma_{k-2}
+// (summing to 1) that minimise || sum_i gamma_i u_i || give the estimate
+// sum_i gamma_i x_i of the limit; it is exact for x_j = x* + sum_{l < k-2}
+// c_l r_l^j, a sequence converging through k-2 geometric modes.  As Sidi
+// recommends, U = QR by modified Gram-Schmidt with one reorthogonalisation
+// pass, then R^H R d = 1 by two triangular solves and gamma = d / sum d;
+// forming U^H U instead squares the condition number.  A column that is
+// dependent on the previous ones to 1e-14 of the largest gets that as its
+// diagonal, which keeps the solves defined.
+typedef std::complex<double> cplx;
+
+// <a, b> over the index ranges of two vectors held as complex arrays
+cplx rre_dot(const cplx *a, const cplx *b, long n) {
+    double re = 0.0, im = 0.0;
+    #pragma omp parallel for reduction(+ : re, im) schedule(static)
+    for (long e = 0; e < n; ++e) {
+        const cplx p = std::conj(a[e]) * b[e];
+        re += p.real();
+        im += p.imag();
+    }
+    return cplx(re, im);
+}
+
+// x: k vectors (pointers) restricted to the ranges [lo[r], hi[r]); g: weights.
+bool rre_weights_of(const std::vector<const double2 *> &x, const std::vector<long> &lo, const std::vector<long> &hi,
+                    std::vector<cplx> &g) {
+    const int k = (int)x.size(), m = k - 1;
+    long n = 0;
+    for (size_t r = 0; r < lo.size(); ++r) n += hi[r] - lo[r];
+    std::vector<std::vector<cplx>> q(m, std::vector<cplx>(n));
+    for (int i = 0; i < m; ++i) {   // gather the differences
+        long o = 0;
+        for (size_t r = 0; r < lo.size(); ++r) {
+            const long len = hi[r] - lo[r];
+            cplx *d = q[i].data() + o;
+            const double2 *a = x[i] + lo[r], *b = x[i + 1] + lo[r];
+            #pragma omp parallel for schedule(static)
+            for (long e = 0; e < len; ++e) d[e] = cplx(b[e].x - a[e].x, b[e].y - a[e].y);
+            o += len;
+        }
+    }
+    std::vector<cplx> R((size_t)m * m, cplx(0.0, 0.0));
+    double rmax = 0.0;
+    for (int j = 0; j < m; ++j) {
+        cplx *v = q[j].data();
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < j; ++i) {
+                const cplx r = rre_dot(q[i].data(), v, n);
+                R[i * m + j] += r;
+                const cplx *qi = q[i].data();
+                #pragma omp parallel for schedule(static)
+                for (long e = 0; e < n; ++e) v[e] -= r * qi[e];
+            }
+        const double nrm = std::sqrt(rre_dot(v, v, n).real());
+        rmax = std::max(rmax, nrm);
+        const double diag = std::max(nrm, 1e-14 * rmax);
+        if (!(diag > 0.0)) return false;
+        R[j * m + j] = diag;
+        const double inv = 1.0 / diag;
+        #pragma omp parallel for schedule(static)
+        for (long e = 0; e < n; ++e) v[e] *= inv;
+    }
+    // R^H y = 1, then R d = y
+    std::vector<cplx> y(m), d(m);
+    for (int i = 0; i < m; ++i) {
+        cplx t(1.0, 0.0);
+        for (int j = 0; j < i; ++j) t -= std::conj(R[j * m + i]) * y[j];
+        y[i] = t / std::conj(R[i * m + i]);
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        cplx t = y[i];
+        for (int j = i + 1; j < m; ++j) t -= R[i * m + j] * d[j];
+        d[i] = t / R[i * m + i];
+    }
+    cplx sum(0.0, 0.0);
+    for (int i = 0; i < m; ++i) sum += d[i];
+    if (!(std::abs(sum) > 0.0)) return false;
+    g.resize(m);
+    for (int i = 0; i < m; ++i) g[i] = d[i] / sum;
+    return true;
+}
+
 // The incident wave e^{i khat.(r - r_inj)} (PSTD_QScat.cpp:624-653, phase
 // per axis (i - i0) d khat_a as at :983-990) evaluated at the grid centre.
 void incident_amplitude(const pstd_params &p, const Geometry &g, double out[2]) {
@@ -793,6 +877,124 @@ int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long 
     }
     *periods_out = k;
     *converged_out = conv ? 1 : 0;
+    return 0;
+}
+
+int pstd_rre_weights(const double *x, long n, int k, double *gamma_out) {
+    if (!x || !gamma_out || n < 1 || k < 3 || k > PSTD_RRE_MAX_RANK + 2) return fail(PSTD_EINVAL, "bad RRE arguments");
+    std::vector<const double2 *> v(k);
+    for (int i = 0; i < k; ++i) v[i] = reinterpret_cast<const double2 *>(x) + (size_t)i * n;
+    std::vector<cplx> g;
+    if (!rre_weights_of(v, {0}, {n}, g)) return fail(PSTD_EINVAL, "RRE system is singular");
+    for (int i = 0; i < k - 1; ++i) {
+        gamma_out[2 * i] = g[i].real();
+        gamma_out[2 * i + 1] = g[i].imag();
+    }
+    return 0;
+}
+
+int pstd_run_to_steady_state_rre(pstd_solver *s, double tol, long max_periods, int rank, long *periods_out,
+                                 double *history, double *raw_history, int *converged_out) {
+    if (!s || !periods_out || !converged_out || max_periods < 1 || !(tol > 0.0) || rank < 1 ||
+        rank > PSTD_RRE_MAX_RANK)
+        return fail(PSTD_EINVAL, "bad steady-state arguments");
+    *periods_out = 0;
+    *converged_out = 0;
+    const long nper = pstd_steps_per_period(s);
+    long nacc = 0;
+    for (int f = 0; f < 6; ++f) nacc += 3 * s->plane_n[f];
+    // the change measure and the RRE inner products run over the surface psi
+    // of the six faces (the first of each face's three planes)
+    std::vector<long> lo(6), hi(6);
+    for (int f = 0; f < 6; ++f) {
+        lo[f] = s->plane_off[f];
+        hi[f] = s->plane_off[f] + s->plane_n[f];
+    }
+    auto psi_norm2 = [&](const double2 *a, const double2 *b) {   // || a - b ||^2 over surface psi (b may be null)
+        double r = 0.0;
+        for (int f = 0; f < 6; ++f) {
+            #pragma omp parallel for reduction(+ : r) schedule(static)
+            for (long e = lo[f]; e < hi[f]; ++e) {
+                const double dx = a[e].x - (b ? b[e].x : 0.0), dy = a[e].y - (b ? b[e].y : 0.0);
+                r += dx * dx + dy * dy;
+            }
+        }
+        return r;
+    };
+    const int keep = rank + 2;
+    std::vector<std::vector<double2>> ring;
+    std::vector<double2> est, prev_est;
+    std::vector<cplx> g;
+    bool conv = false, have_est = false;   // have_est: an estimate from the previous period
+    long k = 0;
+    // window k+1 runs on the device while the host extrapolates from window k
+    int rc = pstd_reset_surface_terms(s);
+    if (!rc) rc = pstd_update_accumulate(s, nper);
+    if (rc) return rc;
+    for (; k < max_periods && !conv; ++k) {
+        std::vector<double2> w;
+        if ((rc = fetch_acc(s, w))) return rc;
+        if (k + 1 < max_periods) {
+            rc = pstd_reset_surface_terms(s);
+            if (!rc) rc = pstd_update_accumulate(s, nper);
+            if (rc) return rc;
+        }
+        if ((long)ring.size() == keep) ring.erase(ring.begin());
+        ring.push_back(std::move(w));
+        const double2 *cur = ring.back().data();
+        double raw = 1.0;
+        if (ring.size() >= 2) {
+            const double a2 = psi_norm2(cur, nullptr);
+            raw = a2 > 0.0 ? std::sqrt(psi_norm2(cur, ring[ring.size() - 2].data()) / a2) : 1.0;
+        }
+        if (raw_history) raw_history[k] = raw;
+        double change = raw;
+        if (ring.size() >= 3) {
+            std::vector<const double2 *> v;
+            for (auto &r : ring) v.push_back(r.data());
+            const int m = (int)ring.size() - 1;
+            if (rre_weights_of(v, lo, hi, g)) {
+                // the estimate over the surface psi, to measure its change
+                est.assign(nacc, double2{0.0, 0.0});
+                for (int f = 0; f < 6; ++f) {
+                    #pragma omp parallel for schedule(static)
+                    for (long e = lo[f]; e < hi[f]; ++e) {
+                        cplx acc(0.0, 0.0);
+                        for (int i = 0; i < m; ++i) acc += g[i] * cplx(v[i][e].x, v[i][e].y);
+                        est[e] = double2{acc.real(), acc.imag()};
+                    }
+                }
+                if (have_est) {
+                    const double a2 = psi_norm2(est.data(), nullptr);
+                    change = a2 > 0.0 ? std::sqrt(psi_norm2(est.data(), prev_est.data()) / a2) : 1.0;
+                    if (change <= tol) conv = true;
+                }
+                prev_est.swap(est);
+                have_est = true;
+            }
+        }
+        if (history) history[k] = change;
+    }
+    *periods_out = k;
+    *converged_out = conv ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(s->stream), "sync");   // a window started past the estimate ends unused
+    // leave the estimate of every accumulator (psi, PS, FD) from the final
+    // windows on the device, normalised as one window of nper steps; when
+    // no estimate exists the last window stays as it is
+    std::vector<const double2 *> v;
+    for (auto &r : ring) v.push_back(r.data());
+    const int m = (int)ring.size() - 1;
+    if (ring.size() >= 3 && rre_weights_of(v, lo, hi, g)) {
+        std::vector<double2> all(nacc);
+        #pragma omp parallel for schedule(static)
+        for (long e = 0; e < nacc; ++e) {
+            cplx acc(0.0, 0.0);
+            for (int i = 0; i < m; ++i) acc += g[i] * cplx(v[i][e].x, v[i][e].y);
+            all[e] = double2{acc.real(), acc.imag()};
+        }
+        HIPCHK(hipMemcpy(s->surf_acc, all.data(), nacc * 16, hipMemcpyHostToDevice), "RRE estimate");
+        s->nSur = nper;
+    }
     return 0;
 }
 

@@ -26,7 +26,8 @@ EXPORTS = [
     "pstd_reset_surface_terms", "pstd_get_psi", "pstd_set_psi", "pstd_set_time_index",
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
-    "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_set_cache_streaming", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
+    "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_set_cache_streaming",
+    "pstd_run_to_steady_state_rre", "pstd_rre_weights", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
 ]
@@ -105,6 +106,10 @@ def library():
         L.pstd_save_psi_plane.argtypes = [vp, i, l, C.c_char_p]
         L.pstd_run_to_steady_state.restype = i
         L.pstd_run_to_steady_state.argtypes = [vp, C.c_double, l, C.POINTER(l), dp, C.POINTER(i)]
+        L.pstd_run_to_steady_state_rre.restype = i
+        L.pstd_run_to_steady_state_rre.argtypes = [vp, C.c_double, l, i, C.POINTER(l), dp, dp, C.POINTER(i)]
+        L.pstd_rre_weights.restype = i
+        L.pstd_rre_weights.argtypes = [dp, l, i, dp]
         L.pstd_batch_create.restype = i
         L.pstd_batch_create.argtypes = [i, C.POINTER(PstdParams), C.POINTER(dp), C.POINTER(sz), C.POINTER(vp)]
         for name in ("pstd_batch_update", "pstd_batch_update_accumulate"):
@@ -143,6 +148,16 @@ def compute_geometry(p):
     g = PstdGeometry()
     _check(library().pstd_compute_geometry(C.byref(p), C.byref(g)))
     return g
+
+
+def rre_weights(x):
+    """Reduced-rank extrapolation weights of the rows of x (k x n complex,
+    k = 3 .. 18): k - 1 complex weights summing to 1 (pstd_rre_weights)."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+    k, n = a.shape
+    out = np.zeros(2 * (k - 1))
+    _check(library().pstd_rre_weights(a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), n, k, _dp(out)))
+    return out[0::2] + 1j * out[1::2]
 
 
 def line_operator(p, axis, seg, kind):
@@ -285,6 +300,19 @@ class PSTDQScat:
                                                   _dp(hist), C.byref(conv)))
         n = periods.value
         return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy()
+
+    def run_to_steady_state_rre(self, tol=1e-6, max_periods=100, rank=12):
+        """Back-to-back windows with reduced-rank extrapolation of the window
+        sequence (pstd_run_to_steady_state_rre): stops when successive
+        estimates differ by <= tol.  Returns (converged, periods, steps,
+        history, raw_history); the accumulators hold the estimate."""
+        hist = np.zeros(int(max_periods))
+        raw = np.zeros(int(max_periods))
+        periods, conv = C.c_long(0), C.c_int(0)
+        _check(library().pstd_run_to_steady_state_rre(self.h, float(tol), int(max_periods), int(rank),
+                                                      C.byref(periods), _dp(hist), _dp(raw), C.byref(conv)))
+        n = periods.value
+        return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy(), raw[:n].copy()
 
     def get_surfaces(self, deriv=PSTD_DERIV_PS):
         n = surface_body_len(self.N)

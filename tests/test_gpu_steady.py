@@ -82,3 +82,55 @@ def test_steady_state_fp32_history_close_to_fp64_oracle():
     oconv, ohist = oracle_history(O.Oracle(p64, V64), s.N, 0.05, 10)
     assert conv == oconv and periods == len(ohist)
     np.testing.assert_allclose(hist, ohist, atol=1e-4)
+
+
+def test_rre_steady_state_closer_than_plain_windows():
+    """Reduced-rank extrapolation of the window sequence on the 96^3
+    Gaussian well (no bound state): after 40 periods the estimate is at
+    least 4x closer to the 150th plain window than the 40th plain window is,
+    the accumulators hold that estimate (all derivative kinds, one-window
+    normalisation), and the plain change history equals the plain run's."""
+    from test_gpu_farfield import _gaussian_problem
+    p, V = _gaussian_problem(-0.5, 1.2)
+    print("case RRE steady state, 96^3 Gaussian well, 40 vs 150 periods", flush=True)
+    s = pstd_amd.PSTDQScat(p, V)
+    nper = s.IterationsToAccumulateSurfaceTerms()
+    plain = {}
+    for k in range(1, 151):
+        s.reset_surface_terms()
+        s.update_accumulate(nper)
+        if k in (39, 40, 150):
+            plain[k] = (s.get_surfaces(0), s.get_surfaces(1))
+    s.close()
+    ref = plain[150][0]
+    g = pstd_amd.PSTDQScat(p, V)
+    conv, periods, steps, hist, raw = g.run_to_steady_state_rre(tol=1e-12, max_periods=40, rank=10)
+    assert not conv and periods == 40 and steps == 40 * nper
+    est_ps, est_fd = g.get_surfaces(0), g.get_surfaces(1)
+    e_rre, e_plain = rel(est_ps, ref), rel(plain[40][0], ref)
+    print(f"error vs window 150: RRE {e_rre:.2e}, plain window 40 {e_plain:.2e}; "
+          f"estimate change {hist[-1]:.2e}, plain change {raw[-1]:.2e}", flush=True)
+    assert e_rre < e_plain / 4
+    # the same weights act on the derivative planes: FD moves with PS
+    assert rel(est_fd, plain[150][1]) < rel(plain[40][1], plain[150][1]) / 4
+    # the plain change of the last window is the change between windows 39 and 40
+    assert abs(raw[-1] - _psi_change(plain[40][0], plain[39][0], g.N)) < 1e-9
+    g.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _psi_change(a, b, N):
+    """Relative L2 change of the surface psi planes between two bodies (the
+    detector's measure: every psi plane of the six faces)."""
+    Nx, Ny, Nz = N
+    dims = [Ny * Nz, Nx * Nz, Nx * Ny]
+    ia, off = [], 0
+    for ax in range(3):
+        for _ in range(2):
+            ia.append(np.arange(off, off + 2 * dims[ax]))
+            off += 4 * dims[ax]
+    idx = np.concatenate(ia)
+    return float(np.linalg.norm(a[idx] - b[idx]) / np.linalg.norm(a[idx]))

@@ -1,0 +1,63 @@
+"""CPU tests of the reduced-rank extrapolation weights (pstd_rre_weights,
+host code of libpstd_amd.so) that pstd_run_to_steady_state_rre applies to
+the window sequence.  RRE is exact for a sequence that converges through at
+most k - 2 geometric modes, x_j = x* + sum_l c_l r_l^j, whatever the
+complex ratios r_l: the estimate sum_i gamma_i x_i must return x* to
+rounding.  With fewer windows than modes + 2 it is not exact but still
+closer than the last term for a dominant slow mode.  A numpy restatement of
+the same least-squares problem gives the same weights."""
+import numpy as np
+import pytest
+
+import pstd_amd
+
+
+def _sequence(nmodes, k, n=4000, seed=0):
+    rng = np.random.default_rng(seed)
+    xs = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c = rng.standard_normal((nmodes, n)) + 1j * rng.standard_normal((nmodes, n))
+    # distinct ratios: phases spread over the circle, moduli 0.6 .. 0.97
+    r = np.linspace(0.97, 0.6, nmodes) * np.exp(2j * np.pi * (np.arange(nmodes) + 0.3) / nmodes)
+    x = np.array([xs + (r[:, None] ** j * c).sum(axis=0) for j in range(k)])
+    return xs, x
+
+
+@pytest.mark.parametrize("nmodes", [1, 3, 6])
+def test_exact_for_geometric_modes(nmodes):
+    xs, x = _sequence(nmodes, nmodes + 2)
+    g = pstd_amd.rre_weights(x)
+    assert abs(g.sum() - 1) < 1e-12
+    est = g @ x[:-1]
+    err = np.linalg.norm(est - xs) / np.linalg.norm(xs)
+    assert err < 1e-8, err
+    assert np.linalg.norm(x[-1] - xs) / np.linalg.norm(xs) > 1e-2      # the sequence itself is far off
+
+
+def test_matches_numpy_least_squares():
+    _, x = _sequence(8, 6, seed=3)
+    g = pstd_amd.rre_weights(x)
+    U = np.diff(x, axis=0).T                       # n x (k-1)
+    # min ||U gamma|| subject to sum gamma = 1, by eliminating the last weight
+    A = U[:, :-1] - U[:, -1:]
+    y = np.linalg.lstsq(A, -U[:, -1], rcond=None)[0]
+    ref = np.append(y, 1 - y.sum())
+    assert abs(g.sum() - 1) < 1e-12
+    assert np.allclose(g, ref, rtol=1e-9, atol=1e-12)
+    # the minimum itself: no other unit-sum weights give a smaller residual
+    res = np.linalg.norm(U @ g)
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        p = rng.standard_normal(len(g)) * 1e-3
+        p -= p.mean()
+        assert np.linalg.norm(U @ (g + p)) >= res * (1 - 1e-12)
+
+
+def test_too_few_modes_still_improves():
+    xs, x = _sequence(5, 5)                         # 5 modes, 3 differences
+    est = pstd_amd.rre_weights(x) @ x[:-1]
+    assert np.linalg.norm(est - xs) < np.linalg.norm(x[-1] - xs)
+
+
+def test_rejects_bad_sizes():
+    with pytest.raises(pstd_amd.PSTDError):
+        pstd_amd.rre_weights(np.ones((2, 10), complex))
