@@ -51,3 +51,26 @@ def test_batch_rejects_long_lines():
     with pytest.raises(pstd_amd.PSTDError) as e:
         pstd_amd.PSTDBatch([p], [V])
     assert e.value.code == pstd_amd.PSTD_EINVAL
+
+
+def test_streamed_batch_matches_solo_runs():
+    """The 48-problem benchmark batch (grids 64..160^3, FP32, 113 M points):
+    its psi and psi^(n-1) together exceed the last-level cache, so the batch
+    runs the streamed-psi^(n-1) kernels; members of every grid size must
+    match their solo runs, which at their own sizes take the plain kernels."""
+    probs = W.config_batch48()
+    batch = pstd_amd.PSTDBatch([q for q, _ in probs], [v for _, v in probs])
+    npts = sum(int(np.prod(m.N)) for m in batch.members)
+    print(f"case streamed batch48, {npts} points", flush=True)
+    assert 2 * npts * 2 * 4 > 256 * 2 ** 20
+    pick = {}
+    for i, m in enumerate(batch.members):
+        pick.setdefault(m.N[0], i)                  # one member per grid size
+    solo = {i: pstd_amd.PSTDQScat(*probs[i]) for i in pick.values()}
+    batch.Update(5)
+    batch.update_accumulate(2)
+    for i, s in solo.items():
+        s.Update(5)
+        s.update_accumulate(2)
+        assert rel(batch[i].get_psi(0), s.get_psi(0)) < 1e-6, i
+        assert rel(batch[i].get_surfaces(0), s.get_surfaces(0)) < 1e-6, i
