@@ -284,7 +284,8 @@ int pstd_synchronize(pstd_solver *s);
  * HIP events around every kernel launch on the solver's stream; write the
  * average duration in milliseconds per launch of each kernel to ms_out:
  * [0] x pass, [1] y pass, [2] z pass (+ potential, TF/SF, ABC),
- * [3] surface capture (one launch per axis).  The state advances as with
+ * [3] surface capture (one launch per axis).  With the y and z passes
+ * fused (pstd_set_pass_fusion), [1] is 0 and [2] is the fused kernel.  The state advances as with
  * pstd_update / pstd_update_accumulate.  No reference counterpart (the
  * reference times with MPI_Wtime, qscat.cpp:165-172). */
 int pstd_time_kernels(pstd_solver *s, long nsteps, long naccum, double ms_out[4]);
@@ -304,6 +305,19 @@ int pstd_set_method(pstd_solver *s, int axis, int method);
  * identical either way; only the speed differs.  (No counterpart in the
  * reference.) */
 int pstd_set_cache_streaming(pstd_solver *s, int mode);
+
+/* Fusion of the y and z passes into one kernel, which adds both axis terms
+ * in registers and reads and writes psi^{n-1} once for the two (the
+ * reference's UpdateY and UpdateZ, PSTD_QScat.cpp:1476-1537, each sweep the
+ * whole box).  mode 1 fuses whenever both axes run the GEMM method, 0
+ * never, -1 (the default) in FP32 only, where the fused kernel is faster;
+ * in FP64 it measures the same as the separate passes.  xrun > 0 sets the
+ * x planes per block of the fused kernel (0: automatic).  Results agree
+ * with the unfused passes to rounding (the two terms are summed before,
+ * not after, their addition to psi^{n-1}).  pstd_get_pass_fusion returns 1
+ * when the next Update runs fused. */
+int pstd_set_pass_fusion(pstd_solver *s, int mode, int xrun);
+int pstd_get_pass_fusion(const pstd_solver *s);
 int pstd_get_method(const pstd_solver *s, int axis);
 
 /* Autotuner: for each axis, time every available method's pass (reps

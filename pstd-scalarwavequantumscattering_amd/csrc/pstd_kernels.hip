@@ -663,6 +663,230 @@ __global__ void k_tick(int *step) {
 }
 
 // ---------------------------------------------------------------------------
+// y and z passes fused.  A block serves one 16-line y block (block yb of y
+// segment sy) against one z segment sz: wave w owns the 16 (y) x 16 (z)
+// output tile of z block w, for a run of x planes.  Per plane
+//     U[y][z] = sum_k A_y[y][k] psi[x, yfft + k, z]   (operator A, psi B)
+//             + sum_k psi[x, y, zfft + k] A_z[z][k]   (psi A, operator B)
+// accumulates in one MFMA accumulator: both products leave row y, column z
+// in the same lane and element, so the two axis terms add in registers and
+// the z-pass epilogue (potential, TF/SF source, ABC) reads and writes psi0
+// once for both axes.  The block's 16 z-lines of the plane -- the z-term
+// operand all its waves share -- are staged in LDS by coalesced loads
+// issued one plane ahead, and so are the y operator fragments of the block's
+// y block; each wave keeps its z operator fragments in registers and streams
+// its y-term operand (4 rows x 16 consecutive z per load) from L2.  Against
+// separate y and z passes this saves one read and one write of psi0 and one
+// read of psi per point and step.  The x term is already in psi0.
+// ---------------------------------------------------------------------------
+// Per wave: D operand loads in flight (a ring, fenced from the scheduler
+// between steps) and EQ rows per epilogue batch; FP64 keeps 40 registers of
+// z operator fragments, FP32 20.
+template <typename T> struct YZDepth;
+template <> struct YZDepth<double> {
+    static constexpr int D = 4, EQ = 2;
+};
+template <> struct YZDepth<float> {
+    static constexpr int D = 8, EQ = 4;
+};
+template <typename T> __host__ __device__ size_t yz_lds_bytes(int KBy, int KBz) {
+    return align16((size_t)16 * (4 * KBz + 1) * 2 * sizeof(T)) + (size_t)KBy * 64 * sizeof(T);
+}
+// KBY, KBZ: compile-time k blocks, for L = 4 KB exactly (no k guards); 0:
+// any KB <= KBMAX read at run time.
+template <typename T, int KBY, int KBZ, bool NT>
+__global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
+    typedef typename VT<T>::c2 C2;
+    typedef typename VT<T>::acc ACC;
+    constexpr int KYM = KbShape<KBY>::KBM, KZM = KbShape<KBZ>::KBM;
+    constexpr int NLD = (16 * 4 * KZM + 255) / 256;   // staging loads per thread (>= 256 threads)
+    constexpr int D = YZDepth<T>::D, EQ = YZDepth<T>::EQ;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int KBy = KBY ? KBY : e.KBy, KBz = KBZ ? KBZ : a.KB;
+    const int Kp = 4 * KBz, LS = Kp + 1;   // staged line stride, odd in complex units
+    C2 *Zs = reinterpret_cast<C2 *>(smem);
+    T *Fy = reinterpret_cast<T *>(smem + align16((size_t)16 * LS * sizeof(C2)));
+    // block -> (x run, unit = (y block, z segment)).  Blocks are dealt to the
+    // 8 XCDs round robin; the units of one run take consecutive slots of one
+    // XCD, so the halo re-reads of neighbouring units meet in its L2.
+    const int b = blockIdx.x, j = b >> 3;
+    const int unit = j % e.nunits, run = (j / e.nunits) * 8 + (b & 7);
+    if (run >= e.nruns) return;   // the whole block leaves
+    int yb = unit / a.seg.n;
+    const int sz = unit - yb * a.seg.n;
+    int sy = 0;
+    while (yb >= e.sy.mb[sy]) yb -= e.sy.mb[sy++];
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, zb = tid >> 6;
+    const int kq = lane >> 4, cq = lane & 15;
+    const int yfft = e.sy.fft1[sy], ycore = e.sy.core1[sy], yn = e.sy.ncore[sy];
+    const int zfft = a.seg.fft1[sz], zcore = a.seg.core1[sz], zn = a.seg.ncore[sz];
+    const bool active = zb < a.seg.mb[sz];
+    {
+        auto opy = glb(e.opy) + e.sy.opoff[sy] + (long)yb * KBy * 64;
+        for (int i = tid; i < KBy * 64; i += nthr) Fy[i] = opy[i];
+    }
+    auto opz = glb(a.op) + a.seg.opoff[sz] + (long)(active ? zb : 0) * KBz * 64 + lane;
+    T fz[KZM];
+#pragma unroll
+    for (int kb = 0; kb < KZM; ++kb) fz[kb] = (KBZ || kb < KBz) ? opz[kb * 64] : T(0);
+    const int Ny = a.Ny, Nz = a.Nz;
+    const long plane = (long)Ny * Nz;
+    auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
+    auto psi0 = glb(reinterpret_cast<C2 *>(a.psi0));
+    auto vbox = glb(a.vbox);
+    // staged line l = y row ycore + yb 16 + l (rows past the core repeat the
+    // core's last row; their results are not stored); columns zfft + 0..Kp-1,
+    // zero past Lz
+    const int rows = min(16, yn - yb * 16), yrow0 = ycore + yb * 16;
+    const int nst = 16 * Kp;
+    C2 pre[NLD];
+    auto fetch = [&](int x) {
+        const auto px = psi + (long)x * plane + zfft;
+#pragma unroll
+        for (int jj = 0; jj < NLD; ++jj) {
+            const int i = tid + jj * nthr;
+            const int line = i / Kp, l = i - line * Kp;
+            C2 v = czero<C2>();
+            if (i < nst && (KBZ || l < a.L)) v = ldv(px + (long)(yrow0 + min(line, rows - 1)) * Nz + l);
+            pre[jj] = v;
+        }
+    };
+    // the lane's output column (and y-term operand column)
+    const bool lcol = active && zb * 16 + cq < zn;
+    const int zc = zcore + (active ? zb * 16 : 0) + (lcol ? cq : 0);
+    const ZInfo zi = z_info<T>(a, zc);
+    const int nyv = a.vb2[1] - a.vb1[1] + 1, nzv = a.vb2[2] - a.vb1[2] + 1;
+    const int offy = (yfft + kq) * Nz + zc, nz4 = 4 * Nz;
+    const int x1 = run * e.xrun, x2 = min(e.Nx, x1 + e.xrun);
+    if (x1 < x2) fetch(x1);
+    for (int x = x1; x < x2; ++x) {
+        __syncthreads();   // the previous plane's staged lines are consumed
+#pragma unroll
+        for (int jj = 0; jj < NLD; ++jj) {
+            const int i = tid + jj * nthr;
+            if (i < nst) {
+                const int line = i / Kp, l = i - line * Kp;
+                Zs[line * LS + l] = pre[jj];
+            }
+        }
+        __syncthreads();
+        if (x + 1 < x2) fetch(x + 1);
+        if (!active) continue;
+        const auto px = psi + (long)x * plane;
+        ACC ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+        // one software-pipelined sweep: KYM y steps (operand from L2, A_y
+        // from LDS), then KZM z steps (operand from LDS, A_z in registers)
+        int oy = offy;   // opaque to loop-invariant code motion: hoisted out
+        asm volatile("" : "+v"(oy));   // of the x loop, the KYM addresses would stay live
+        const C2 *zrow = Zs + cq * LS + kq;
+        auto load = [&](int st) {
+            C2 v = czero<C2>();
+            if (st < KYM) {
+                if (KBY || (st < KBy && 4 * st + kq < e.Ly)) v = ldv(px + (oy + st * nz4));
+            } else if (KBZ || st - KYM < KBz) {
+                v = zrow[4 * (st - KYM)];
+            }
+            return v;
+        };
+        constexpr int NS = KYM + KZM;
+        C2 ring[D];
+        T fring[D];
+#pragma unroll
+        for (int st = 0; st < D; ++st) {
+            ring[st] = load(st);
+            fring[st] = (st < KYM && (KBY || st < KBy)) ? Fy[st * 64 + lane] : T(0);
+        }
+#pragma unroll
+        for (int st = 0; st < NS; ++st) {
+            const C2 v = ring[st % D];
+            const T f = fring[st % D];
+            if (st + D < NS) {
+                ring[st % D] = load(st + D);
+                if (st + D < KYM && (KBY || st + D < KBy)) fring[st % D] = Fy[(st + D) * 64 + lane];
+            }
+            if (st < KYM) {
+                if (KBY || st < KBy) {
+                    ar = mfma16(f, v.x, ar);
+                    ai = mfma16(f, v.y, ai);
+                }
+            } else if (KBZ || st - KYM < KBz) {
+                ar = mfma16(v.x, fz[st - KYM], ar);
+                ai = mfma16(v.y, fz[st - KYM], ai);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // epilogue of the z pass (z_point), with the per-line terms formed
+        // here from (x, y) instead of read from the line table
+        if (!lcol) continue;
+        // the lane's row group and column, opaque to loop-invariant code
+        // motion like the operand offset
+        int kqe = kq, zce = zc;
+        asm volatile("" : "+v"(kqe), "+v"(zce));
+        const double gx = glb(a.g[0])[x];
+        const bool vx = a.use_v && x >= a.vb1[0] && x <= a.vb2[0] && (zi.flags & 1);
+        const bool tx = x >= a.k1a[0] && x <= a.k2b[0], tx_in = x > a.k1b[0] && x < a.k2a[0];
+        // the rows in batches of EQ: all loads of a batch first (rows past
+        // the core read the core's last row and are not stored), then the
+        // arithmetic
+#pragma unroll
+        for (int q0 = 0; q0 < 4; q0 += EQ) {
+            C2 acc[4], pn[4];
+            T vv[4];
+#pragma unroll
+            for (int q = q0; q < q0 + EQ; ++q) {
+                const int r = min(yb * 16 + VT<T>::row(kqe, q), yn - 1);
+                const int y = ycore + r;
+                const long pt = (long)y * Nz + zce;
+                acc[q] = ld_psi0<NT>(psi0 + (long)x * plane + pt);
+                vv[q] = T(0);
+                pn[q] = czero<C2>();
+                if (vx && y >= a.vb1[1] && y <= a.vb2[1]) {
+                    vv[q] = vbox[((long)(x - a.vb1[0]) * nyv + (y - a.vb1[1])) * nzv + zi.zoff];
+                    pn[q] = ldv(px + pt);
+                }
+            }
+#pragma unroll
+            for (int q = q0; q < q0 + EQ; ++q) {
+                const int r = yb * 16 + VT<T>::row(kqe, q);
+                if (r >= yn) continue;
+                const int y = ycore + r;
+                C2 c = acc[q];
+                c.x += ai[q];
+                c.y -= ar[q];
+                const T v = a.vfac * vv[q];
+                c.x += v * pn[q].y;
+                c.y -= v * pn[q].x;
+                const bool ty = y >= a.k1a[1] && y <= a.k2b[1], ty_in = y > a.k1b[1] && y < a.k2a[1];
+                if ((zi.flags & 2) && tx && ty && !((zi.flags & 4) && tx_in && ty_in)) {
+                    // line terms as line_info forms them at unit time factor ...
+                    const double pxv = glb(a.tfP[0])[x], pyv = glb(a.tfP[1])[y];
+                    const double2 ax = ldv(glb(a.tfA[0]) + x), ay = ldv(glb(a.tfA[1]) + y);
+                    const double2 ex = ldv(glb(a.tfE[0]) + x), ey = ldv(glb(a.tfE[1]) + y);
+                    const double pxy = pxv * pyv;
+                    const double cx = ax.x * pyv + ay.x * pxv, cy = ax.y * pyv + ay.y * pxv;
+                    const double exr = ex.x * ey.x - ex.y * ey.y, exi = ex.x * ey.y + ex.y * ey.x;
+                    // ... and the z terms with the time factor (sin t + i cos t)
+                    // folded into E_z, as the register z pass does
+                    const ZSrc zs = z_src<T>(a, zce);
+                    const double ezr = zs.ez.x * a.sin_t - zs.ez.y * a.cos_t;
+                    const double ezi = zs.ez.x * a.cos_t + zs.ez.y * a.sin_t;
+                    const double ur2 = cx * zs.pz + pxy * zs.az.x;
+                    const double ui2 = cy * zs.pz + pxy * zs.az.y;
+                    const double er = exr * ezr - exi * ezi;
+                    const double ei = exr * ezi + exi * ezr;
+                    c.x -= (T)(er * ur2 - ei * ui2);
+                    c.y -= (T)(er * ui2 + ei * ur2);
+                }
+                const double gg = gx * glb(a.g[1])[y] * zi.gz;
+                c.x = (T)(c.x * gg);
+                c.y = (T)(c.y * gg);
+                st_psi0<NT>(psi0 + (long)x * plane + (long)y * Nz + zce, c);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // LDS-operator variants (any L whose operator and tile fit in LDS).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1106,6 +1330,28 @@ template <typename T> const void *line_reg_kernel(int KB, bool nt) { PSTD_KB_SWI
 template <typename T> const void *batch_plane_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_batch_plane, nt) }
 template <typename T> const void *batch_line_kernel(int KB, bool nt) { PSTD_KB_SWITCH(k_batch_line, nt) }
 #undef PSTD_KB_SWITCH
+// fused y + z pass: compile-time k blocks when both axes have L = 4 KB with
+// the same KB = 16, 20, 24
+template <typename T> const void *yz_kernel(int KBy, int Ly, int KBz, int Lz, bool nt) {
+    switch ((KBy == KBz && Ly == 4 * KBy && Lz == 4 * KBz) ? KBy : 0) {
+    case 16: return nt ? (const void *)k_lap_yz<T, 16, 16, true> : (const void *)k_lap_yz<T, 16, 16, false>;
+    case 20: return nt ? (const void *)k_lap_yz<T, 20, 20, true> : (const void *)k_lap_yz<T, 20, 20, false>;
+    case 24: return nt ? (const void *)k_lap_yz<T, 24, 24, true> : (const void *)k_lap_yz<T, 24, 24, false>;
+    default: return nt ? (const void *)k_lap_yz<T, 0, 0, true> : (const void *)k_lap_yz<T, 0, 0, false>;
+    }
+}
+bool yz_fusable(int MBy, int KBy, int MBz, int KBz) {
+    return reg_variant_ok(MBy, KBy) && reg_variant_ok(MBz, KBz);
+}
+template <typename T>
+hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s) {
+    const long blocks = 8L * e.nunits * ((e.nruns + 7) / 8);
+    LineArgs<T> arg = a;
+    YZArgs<T> ext = e;
+    void *args[] = {&arg, &ext};
+    return hipLaunchKernel(yz_kernel<T>(e.KBy, e.Ly, a.KB, a.L, a.nt != 0), dim3((unsigned)blocks), dim3(64 * e.zw),
+                           args, yz_lds_bytes<T>(e.KBy, a.KB), s);
+}
 
 template <typename T>
 hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s) {
@@ -1270,6 +1516,11 @@ hipError_t configure_kernels(int max_lds) {
             k.push_back(batch_line_kernel<double>(kb, nt));
             k.push_back(batch_line_kernel<float>(kb, nt));
         }
+    for (int kb : {0, 16, 20, 24})
+        for (bool nt : {false, true}) {
+            k.push_back(yz_kernel<double>(kb, 4 * kb, kb, 4 * kb, nt));
+            k.push_back(yz_kernel<float>(kb, 4 * kb, kb, 4 * kb, nt));
+        }
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return e;
@@ -1283,6 +1534,8 @@ template hipError_t launch_line_table<float>(const LineArgs<float> &, void *, hi
 template hipError_t launch_plane<float>(const PlaneArgs<float> &, int, hipStream_t);
 template hipError_t launch_line<double>(const LineArgs<double> &, int, hipStream_t);
 template hipError_t launch_line<float>(const LineArgs<float> &, int, hipStream_t);
+template hipError_t launch_yz<double>(const LineArgs<double> &, const YZArgs<double> &, hipStream_t);
+template hipError_t launch_yz<float>(const LineArgs<float> &, const YZArgs<float> &, hipStream_t);
 template hipError_t launch_surface<double>(const double *, const SurfArgs &, long, hipStream_t);
 template hipError_t launch_surface<float>(const float *, const SurfArgs &, long, hipStream_t);
 template size_t pass_lds_bytes<double>(int, int);

@@ -87,6 +87,20 @@ struct LineArgs {
     FftPlan<T> fft;
 };
 
+// y and z passes fused into one kernel (launch_yz): the z pass's LineArgs
+// (z operator and segments, the pointwise terms) plus the y operator.  A
+// block serves one unit -- a 16-row y block against a z segment, one wave
+// per 16-row z block -- for a run of xrun consecutive x planes.
+template <typename T>
+struct YZArgs {
+    const T *opy;
+    SegTable sy;
+    int Nx, Ly, KBy;
+    int nunits;           // (16-row y blocks summed over the y segments) x (z segments)
+    int zw;               // waves per block: max(4, 16-row blocks of the largest z segment)
+    int xrun, nruns;      // x planes per block, runs = ceil(Nx / xrun)
+};
+
 struct SurfFace {
     long stride_l, stride_b, stride_c;   // complex elements
     long fft1;                           // global start of the owning segment
@@ -104,6 +118,9 @@ struct SurfArgs {          // the two faces of one axis
 template <typename T> hipError_t launch_plane(const PlaneArgs<T> &a, int grid_x, hipStream_t s);
 template <typename T> hipError_t launch_line(const LineArgs<T> &a, int grid_x, hipStream_t s);
 template <typename T> hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStream_t s);
+// fused y + z pass; needs the register GEMM variant on both axes (yz_fusable)
+template <typename T> hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s);
+bool yz_fusable(int MBy, int KBy, int MBz, int KBz);
 template <typename T> size_t pass_lds_bytes(int MB, int KB);
 template <typename T> int pass_occupancy(int MB, int KB, bool line);
 template <typename T> size_t fft_lds_bytes(int L);

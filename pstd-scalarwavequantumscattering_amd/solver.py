@@ -27,6 +27,7 @@ EXPORTS = [
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
     "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_set_cache_streaming",
+    "pstd_set_pass_fusion", "pstd_get_pass_fusion",
     "pstd_run_to_steady_state_rre", "pstd_rre_weights", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
@@ -98,6 +99,10 @@ def library():
         L.pstd_incident_amplitude.argtypes = [C.POINTER(PstdParams), dp]
         L.pstd_set_cache_streaming.restype = i
         L.pstd_set_cache_streaming.argtypes = [vp, i]
+        L.pstd_set_pass_fusion.restype = i
+        L.pstd_set_pass_fusion.argtypes = [vp, i, i]
+        L.pstd_get_pass_fusion.restype = i
+        L.pstd_get_pass_fusion.argtypes = [vp]
         L.pstd_set_file_precision.restype = i
         L.pstd_set_file_precision.argtypes = [vp, i]
         L.pstd_save_g.restype = i
@@ -271,6 +276,16 @@ class PSTDQScat:
     def set_cache_streaming(self, mode):
         """psi^{n-1} accesses with non-temporal hints (1), plain (0) or by working set (-1)."""
         _check(library().pstd_set_cache_streaming(self.h, int(mode)))
+
+    def set_pass_fusion(self, mode, xrun=0):
+        """y and z passes as one kernel: 1 wherever both run the GEMM, 0
+        never, -1 (the default) in FP32 only; xrun = x planes per block
+        (0: automatic)."""
+        _check(library().pstd_set_pass_fusion(self.h, int(mode), int(xrun)))
+
+    def pass_fusion(self):
+        """1 when the next Update runs the fused y + z kernel."""
+        return library().pstd_get_pass_fusion(self.h)
 
     def set_file_precision(self, precision):
         """Surface files as FP64 (8) or, like the reference's FP32 build, FP32 (4)."""

@@ -245,3 +245,52 @@ def test_cache_streaming_is_bit_identical(precision):
     with pytest.raises(pstd_amd.PSTDError):
         g.set_cache_streaming(2)
     g.close()
+
+
+@pytest.mark.parametrize("precision", [pstd_amd.PSTD_FP64, pstd_amd.PSTD_FP32])
+@pytest.mark.parametrize("cfg", CASES)
+def test_fused_yz_matches_separate_passes(cfg, precision):
+    """The fused y + z kernel (the default in FP32) against the separate y
+    and z passes (the default in FP64), three steps from the same random
+    state inside the TF/SF and potential terms' time window: the paths differ
+    only in summing the y and z terms before adding them to psi^{n-1}, so
+    they agree to rounding (FP64 1e-13, FP32 5e-6).  Every x run length
+    gives bit-identical results (the per-point arithmetic does not depend on
+    it)."""
+    p, V = W.config_tiny(precision=precision, **cfg)
+    print(f"case fused y+z nprocs={tuple(p.nprocs)} L={tuple(p.L)} prec={precision}", flush=True)
+    n = int(np.prod(list(pstd_amd.compute_geometry(p).N)))
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    fp32 = precision == pstd_amd.PSTD_FP32
+    out = {}
+    for mode, xrun in ((0, 0), (-1, 0), (1, 0), (1, 1), (1, 3), (1, 1000)):
+        g = pstd_amd.PSTDQScat(p, V)
+        g.set_pass_fusion(mode, xrun)
+        assert g.pass_fusion() == (int(fp32) if mode == -1 else mode)
+        g.set_psi(0, a)
+        g.set_psi(1, b)
+        g.set_time_index(700)
+        g.Update(3)
+        out[(mode, xrun)] = g.get_psi(0)
+        g.close()
+    tol = 1e-13 if precision == pstd_amd.PSTD_FP64 else 5e-6
+    assert rel(out[(1, 0)], out[(0, 0)]) < tol
+    assert np.array_equal(out[(-1, 0)], out[(1, 0)] if fp32 else out[(0, 0)])
+    for xrun in (1, 3, 1000):
+        assert np.array_equal(out[(1, xrun)], out[(1, 0)])
+
+
+def test_pass_fusion_argument_checks():
+    p, V = W.config_tiny(**CASES[0])
+    g = pstd_amd.PSTDQScat(p, V)
+    for mode, xrun in ((2, 0), (-2, 0), (0, -1)):
+        with pytest.raises(pstd_amd.PSTDError) as e:
+            g.set_pass_fusion(mode, xrun)
+        assert e.value.code == pstd_amd.PSTD_EINVAL
+    g.set_pass_fusion(1)
+    assert g.pass_fusion() == 1
+    g.set_method(1, pstd_amd.PSTD_METHOD_FFT)
+    assert g.pass_fusion() == 0           # fusion needs the GEMM on y and z
+    g.close()
