@@ -75,8 +75,9 @@ def steady_state(pkg, W, max_periods, err_steps):
     (pstd_run_to_steady_state_rre, rank 12, tol 1e-6, capped at max_periods),
     the first period at which the change between successive estimates -- and,
     from the same run, the plain change between windows -- is below each
-    tolerance; and the FP32 path's error against the FP64 path (parity-equal to
-    the FP64 oracle) after err_steps steps."""
+    tolerance; the same steady state started in FP32 (pstd_run_to_steady_state_mixed)
+    with its distance from the FP64-only estimate; and the FP32 path's error
+    against the FP64 path (parity-equal to the FP64 oracle) after err_steps steps."""
     import numpy as np
     p, V = W.config_grf224(precision=pkg.PSTD_FP64)
     s = pkg.PSTDQScat(p, V)
@@ -86,8 +87,24 @@ def steady_state(pkg, W, max_periods, err_steps):
     t0 = time.perf_counter()
     conv, periods, steps, hist, raw = s.run_to_steady_state_rre(tol=1e-6, max_periods=max_periods, rank=12)
     dt = time.perf_counter() - t0
+    ref = s.get_surfaces(0)
     s.close()
     per = dt / periods
+    # the same steady state started in FP32 (pstd_run_to_steady_state_mixed)
+    s32 = pkg.PSTDQScat(*W.config_grf224(precision=pkg.PSTD_FP32))
+    s64 = pkg.PSTDQScat(p, V)
+    s64.autotune(reps=5)
+    s64.synchronize()
+    t0 = time.perf_counter()
+    mconv, n32, n64, mhist, _ = s64.run_to_steady_state_mixed(s32, switch_tol=1e-4, switch_max=max_periods, tol=1e-6,
+                                                              max_periods=max_periods, rank=12)
+    mdt = time.perf_counter() - t0
+    mixed = {"criterion": "FP32 windows until successive RRE estimates change by <= 1e-4, then FP64 windows from "
+             "that state until they change by <= 1e-6", "converged_1e-6": mconv, "periods_fp32": n32,
+             "periods_fp64": n64, "seconds": round(mdt, 3),
+             "estimate_rel_l2_vs_fp64_only": float(np.linalg.norm(s64.get_surfaces(0) - ref) / np.linalg.norm(ref))}
+    s32.close()
+    s64.close()
 
     def reached(h):
         out = {}
@@ -108,7 +125,7 @@ def steady_state(pkg, W, max_periods, err_steps):
             "change between consecutive windows of the same run alongside", "steps_per_period": nper,
             "periods_run": periods, "converged_1e-6": conv, "last_change": float(hist[-1]),
             "last_plain_change": float(raw[-1]), "seconds_per_period": round(per, 4),
-            "reached": reached(hist), "reached_plain_windows": reached(raw),
+            "reached": reached(hist), "reached_plain_windows": reached(raw), "mixed_fp32_fp64": mixed,
             "fp32_psi_rel_l2_vs_fp64": {"steps": err_steps, "value": float(np.linalg.norm(x - y) / np.linalg.norm(y))}}
 
 

@@ -15,6 +15,7 @@
 //          (L <= 96, core <= 128); the psi tile is double-buffered through
 //          registers so the next tile's loads overlap this tile's MFMAs.
 //   *_lds  operator fragments in LDS; any L whose tiles fit on chip.
+#include <algorithm>
 #include <vector>
 
 #include "pstd_kernels.hpp"
@@ -660,6 +661,23 @@ __global__ void __launch_bounds__(256) k_line_table(LineArgs<T> a, LineInfo *tab
 
 __global__ void k_tick(int *step) {
     if (threadIdx.x == 0) *step += 1;
+}
+
+// FP32 -> FP64 copy of n complex points (a state handed from an FP32 solve to
+// an FP64 solver of the same problem)
+__global__ void __launch_bounds__(256) k_widen(const float2 *src, double2 *dst, long n) {
+    auto in = glb(src);
+    auto out = glb(dst);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float2 v = ldv(in + i);
+        stv(out + i, make_double2(v.x, v.y));
+    }
+}
+hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s) {
+    const long blocks = std::min<long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_widen, dim3((unsigned)std::max<long>(blocks, 1)), dim3(256), 0, s,
+                       reinterpret_cast<const float2 *>(src), reinterpret_cast<double2 *>(dst), n);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -136,6 +136,8 @@ template <typename T>
 hipError_t launch_batch_line(const LineArgs<T> *list, const int2 *map, int nmap, const double *clock, const int *step,
                              int nprob, int grid_x, int block, int KB, bool kb_uniform, bool nt, hipStream_t s);
 hipError_t launch_tick(int *step, hipStream_t s);
+// dst[i] = (double2) src[i], i < n (FP32 field into an FP64 one)
+hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s);
 constexpr size_t LINE_RECORD_BYTES = 64;
 // psi part of the six faces' accumulators: face f's plane_n points start at
 // acc + off[f]; start[f] is its first index in the packed copy (total points)

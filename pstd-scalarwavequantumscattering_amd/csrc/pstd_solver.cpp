@@ -1046,6 +1046,40 @@ int pstd_run_to_steady_state_rre(pstd_solver *s, double tol, long max_periods, i
     return 0;
 }
 
+int pstd_run_to_steady_state_mixed(pstd_solver *s32, pstd_solver *s64, double switch_tol, long switch_max,
+                                   double tol, long max_periods, int rank, long *periods32_out,
+                                   long *periods64_out, double *history, double *raw_history, int *converged_out) {
+    if (!s32 || !s64 || !periods32_out || !periods64_out || !converged_out || switch_max < 1 || !(switch_tol > 0.0))
+        return fail(PSTD_EINVAL, "bad steady-state arguments");
+    if (s32->esz != 4 || s64->esz != 8) return fail(PSTD_EINVAL, "mixed steady state needs an FP32 and an FP64 solver");
+    const pstd_params &a = s32->p, &b = s64->p;
+    bool same = s32->device == s64->device && a.dtau == b.dtau && a.n_abc == b.n_abc && a.n_sca == b.n_sca &&
+                a.n_ksi == b.n_ksi && a.n_halo == b.n_halo && a.theta == b.theta && a.phi == b.phi && a.E0 == b.E0 &&
+                a.cap_U0 == b.cap_U0 && a.cap_alpha == b.cap_alpha && a.profile == b.profile &&
+                s32->has_v == s64->has_v;
+    for (int d = 0; d < 3; ++d)
+        same = same && a.nprocs[d] == b.nprocs[d] && a.d[d] == b.d[d] && a.L[d] == b.L[d];
+    if (!same) return fail(PSTD_EINVAL, "the FP32 and FP64 solvers describe different problems");
+    *periods32_out = *periods64_out = 0;
+    *converged_out = 0;
+    // FP32 windows with extrapolation until the estimate settles to switch_tol
+    long p32 = 0;
+    int conv32 = 0;
+    int rc = pstd_run_to_steady_state_rre(s32, switch_tol, switch_max, rank, &p32, nullptr, nullptr, &conv32);
+    if (rc) return rc;
+    *periods32_out = p32;
+    // hand psi^n, psi^{n-1} and the time index to the FP64 solver
+    HIPCHK(hipStreamSynchronize(s32->stream), "sync");
+    HIPCHK(hipStreamSynchronize(s64->stream), "sync");
+    const long n = s32->npts();
+    HIPCHK(launch_widen(s32->psi[s32->cur], s64->psi[s64->cur], n, s64->stream), "widen psi");
+    HIPCHK(launch_widen(s32->psi[1 - s32->cur], s64->psi[1 - s64->cur], n, s64->stream), "widen psi");
+    s64->nTau = s32->nTau;
+    // FP64 windows with extrapolation to tol
+    return pstd_run_to_steady_state_rre(s64, tol, max_periods, rank, periods64_out, history, raw_history,
+                                        converged_out);
+}
+
 int pstd_set_cache_streaming(pstd_solver *s, int mode) {
     if (!s || mode < -1 || mode > 1) return fail(PSTD_EINVAL, "streaming mode must be -1, 0 or 1");
     s->nt_mode = mode;

@@ -27,7 +27,7 @@ EXPORTS = [
     "pstd_synchronize", "pstd_last_error", "pstd_time_kernels", "pstd_line_operator",
     "pstd_set_method", "pstd_get_method", "pstd_autotune", "pstd_run_to_steady_state",
     "pstd_save_g", "pstd_save_psi_plane", "pstd_set_file_precision", "pstd_set_cache_streaming",
-    "pstd_set_pass_fusion", "pstd_get_pass_fusion",
+    "pstd_set_pass_fusion", "pstd_get_pass_fusion", "pstd_run_to_steady_state_mixed",
     "pstd_run_to_steady_state_rre", "pstd_rre_weights", "pstd_far_field", "pstd_far_field_file", "pstd_incident_amplitude",
     "pstd_batch_create", "pstd_batch_update", "pstd_batch_update_accumulate", "pstd_batch_size",
     "pstd_batch_solver", "pstd_batch_synchronize", "pstd_batch_destroy",
@@ -99,6 +99,9 @@ def library():
         L.pstd_incident_amplitude.argtypes = [C.POINTER(PstdParams), dp]
         L.pstd_set_cache_streaming.restype = i
         L.pstd_set_cache_streaming.argtypes = [vp, i]
+        L.pstd_run_to_steady_state_mixed.restype = i
+        L.pstd_run_to_steady_state_mixed.argtypes = [vp, vp, C.c_double, l, C.c_double, l, i, C.POINTER(l),
+                                                     C.POINTER(l), dp, dp, C.POINTER(i)]
         L.pstd_set_pass_fusion.restype = i
         L.pstd_set_pass_fusion.argtypes = [vp, i, i]
         L.pstd_get_pass_fusion.restype = i
@@ -328,6 +331,22 @@ class PSTDQScat:
                                                       C.byref(periods), _dp(hist), _dp(raw), C.byref(conv)))
         n = periods.value
         return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy(), raw[:n].copy()
+
+    def run_to_steady_state_mixed(self, fp32, switch_tol=1e-4, switch_max=60, tol=1e-6, max_periods=100, rank=12):
+        """This FP64 solver's steady state, started by the FP32 solver fp32 of
+        the same problem (pstd_run_to_steady_state_mixed): fp32 runs
+        extrapolated windows to switch_tol, hands its state over, and this
+        solver runs them to tol.  Returns (converged, periods32, periods64,
+        history, raw_history) of the FP64 phase; the accumulators hold the
+        estimate."""
+        hist = np.zeros(int(max_periods))
+        raw = np.zeros(int(max_periods))
+        p32, p64, conv = C.c_long(0), C.c_long(0), C.c_int(0)
+        _check(library().pstd_run_to_steady_state_mixed(fp32.h, self.h, float(switch_tol), int(switch_max), float(tol),
+                                                        int(max_periods), int(rank), C.byref(p32), C.byref(p64),
+                                                        _dp(hist), _dp(raw), C.byref(conv)))
+        n = p64.value
+        return bool(conv.value), p32.value, n, hist[:n].copy(), raw[:n].copy()
 
     def get_surfaces(self, deriv=PSTD_DERIV_PS):
         n = surface_body_len(self.N)

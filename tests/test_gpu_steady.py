@@ -134,3 +134,44 @@ def _psi_change(a, b, N):
             off += 4 * dims[ax]
     idx = np.concatenate(ia)
     return float(np.linalg.norm(a[idx] - b[idx]) / np.linalg.norm(a[idx]))
+
+
+def test_mixed_precision_steady_state_matches_fp64():
+    """pstd_run_to_steady_state_mixed on the 96^3 Gaussian well: FP32
+    extrapolated windows to a 1e-4 estimate change, then FP64 windows from
+    the handed-over state to 1e-6.  The estimate agrees with the FP64-only
+    run's to 2e-6 (both stop at a 1e-6 change), although the FP32 phase
+    alone is about 1e-4 off; the FP64 solver's clock continues the FP32 one;
+    mismatched solver pairs are rejected."""
+    from test_gpu_farfield import _gaussian_problem
+    p64, V = _gaussian_problem(-0.5, 1.2)
+    p32, _ = _gaussian_problem(-0.5, 1.2, precision=pstd_amd.PSTD_FP32)
+    print("case mixed-precision steady state, 96^3 Gaussian well", flush=True)
+    a = pstd_amd.PSTDQScat(p64, V)
+    conv, n64, _, _, _ = a.run_to_steady_state_rre(tol=1e-6, max_periods=150, rank=10)
+    assert conv
+    ref = (a.get_surfaces(0), a.get_surfaces(1))
+    a.close()
+    s32 = pstd_amd.PSTDQScat(p32, V)
+    s64 = pstd_amd.PSTDQScat(p64, V)
+    nper = s64.IterationsToAccumulateSurfaceTerms()
+    conv, n1, n2, hist, raw = s64.run_to_steady_state_mixed(s32, switch_tol=1e-4, switch_max=80, tol=1e-6,
+                                                             max_periods=150, rank=10)
+    e_ps, e_fd = rel(s64.get_surfaces(0), ref[0]), rel(s64.get_surfaces(1), ref[1])
+    print(f"FP64 alone {n64} periods; mixed {n1} FP32 + {n2} FP64 periods; estimate vs FP64-only: "
+          f"PS {e_ps:.2e} FD {e_fd:.2e}", flush=True)
+    assert conv and n1 > 0 and n2 > 0 and len(hist) == n2 and hist[-1] <= 1e-6
+    assert e_ps < 2e-6 and e_fd < 2e-6
+    t32, t64 = s32.time_index(), s64.time_index()
+    assert t32 > n1 * nper and t64 - t32 in (n2 * nper, (n2 + 1) * nper)
+    with pytest.raises(pstd_amd.PSTDError) as e:      # roles swapped: the FP32 solver as the FP64 one
+        s32.run_to_steady_state_mixed(s64, max_periods=2)
+    assert e.value.code == pstd_amd.PSTD_EINVAL
+    other, Vo = _gaussian_problem(-0.5, 1.2, precision=pstd_amd.PSTD_FP32)
+    other.theta = 0.7                                   # a different incidence
+    o32 = pstd_amd.PSTDQScat(other, Vo)
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        s64.run_to_steady_state_mixed(o32, max_periods=2)
+    assert e.value.code == pstd_amd.PSTD_EINVAL
+    for g in (s32, s64, o32):
+        g.close()
