@@ -327,13 +327,12 @@ int pstd_set_cache_streaming(pstd_solver *s, int mode);
 /* Fusion of the y and z passes into one kernel, which adds both axis terms
  * in registers and reads and writes psi^{n-1} once for the two (the
  * reference's UpdateY and UpdateZ, PSTD_QScat.cpp:1476-1537, each sweep the
- * whole box).  mode 1 fuses whenever both axes run the GEMM method, 0
- * never, -1 (the default) in FP32 only, where the fused kernel is faster;
- * in FP64 it measures the same as the separate passes.  xrun > 0 sets the
- * x planes per block of the fused kernel (0: automatic).  Results agree
- * with the unfused passes to rounding (the two terms are summed before,
- * not after, their addition to psi^{n-1}).  pstd_get_pass_fusion returns 1
- * when the next Update runs fused. */
+ * whole box).  mode -1 (the default) and 1 fuse whenever both axes run the
+ * GEMM method, 0 never.  xrun > 0 sets the x planes per block of the fused
+ * kernel (0: automatic).  Results agree with the unfused passes to rounding
+ * (the two terms are summed before, not after, their addition to
+ * psi^{n-1}; FP64 forms the z operator from its two factors).
+ * pstd_get_pass_fusion returns 1 when the next Update runs fused. */
 int pstd_set_pass_fusion(pstd_solver *s, int mode, int xrun);
 int pstd_get_pass_fusion(const pstd_solver *s);
 int pstd_get_method(const pstd_solver *s, int axis);

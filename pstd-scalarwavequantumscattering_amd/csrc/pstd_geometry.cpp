@@ -246,6 +246,18 @@ void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout 
     }
 }
 
+void build_laplacian_tables(const Axis &a, long double alpha, std::vector<long double> &at,
+                            std::vector<long double> &win) {
+    std::vector<long double> t = d2_row_table(a.L, a.d);
+    at.resize(2 * a.L);
+    for (long j = 0; j < 2 * a.L; ++j) at[j] = alpha * t[j % a.L];
+    win.clear();
+    for (int r = 0; r < (int)a.P; ++r) {
+        std::vector<long double> W = segment_window(a, r);
+        win.insert(win.end(), W.begin(), W.end());
+    }
+}
+
 FftTables build_fft_tables(const Axis &a, long double alpha) {
     FftTables t;
     long m = a.L;

@@ -65,6 +65,12 @@ struct OpLayout {
 OpLayout op_layout(const Axis &a);
 void build_laplacian_fragments(const Axis &a, long double alpha, const OpLayout &lay,
                                std::vector<long double> &frag);
+// The same operator as its two factors, for kernels that form fragments on
+// the fly: at[j] = alpha * D2 row value t[j mod L] for j < 2L (so that
+// A[m][l] = at[i1 + m - l + L] * W(l) needs no modulo), and win = the
+// windows of the P segments, L values each.
+void build_laplacian_tables(const Axis &a, long double alpha, std::vector<long double> &at,
+                            std::vector<long double> &win);
 
 // Stockham FFT tables of an axis (FFT variant of the passes): radices
 // (4, 2, 3, 5, 7 in that order), Ns per stage, twiddles

@@ -698,8 +698,11 @@ hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s) {
 // read of psi per point and step.  The x term is already in psi0.
 // ---------------------------------------------------------------------------
 // Per wave: D operand loads in flight (a ring, fenced from the scheduler
-// between steps) and EQ rows per epilogue batch; FP64 keeps 40 registers of
-// z operator fragments, FP32 20.
+// between steps) and EQ rows per epilogue batch.  FP32 keeps its z operator
+// fragments in registers (20).  FP64 would need 40, which caps it at 3 waves
+// per SIMD; it forms each fragment instead from the operator's two factors,
+// A_z[m][l] = at[i1 + m - l + L] W(l) (a circulant row and the segment's
+// window, staged in LDS): two LDS reads and a multiply per k step.
 template <typename T> struct YZDepth;
 template <> struct YZDepth<double> {
     static constexpr int D = 4, EQ = 2;
@@ -707,8 +710,12 @@ template <> struct YZDepth<double> {
 template <> struct YZDepth<float> {
     static constexpr int D = 8, EQ = 4;
 };
+template <typename T> struct YZGen {
+    static constexpr bool on = sizeof(T) == 8;
+};
 template <typename T> __host__ __device__ size_t yz_lds_bytes(int KBy, int KBz) {
-    return align16((size_t)16 * (4 * KBz + 1) * 2 * sizeof(T)) + (size_t)KBy * 64 * sizeof(T);
+    return align16((size_t)16 * (4 * KBz + 1) * 2 * sizeof(T)) + (size_t)KBy * 64 * sizeof(T) +
+           (YZGen<T>::on ? (size_t)3 * 4 * KBz * sizeof(T) : 0);
 }
 // KBY, KBZ: compile-time k blocks, for L = 4 KB exactly (no k guards); 0:
 // any KB <= KBMAX read at run time.
@@ -722,8 +729,11 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int KBy = KBY ? KBY : e.KBy, KBz = KBZ ? KBZ : a.KB;
     const int Kp = 4 * KBz, LS = Kp + 1;   // staged line stride, odd in complex units
+    constexpr bool GEN = YZGen<T>::on;
     C2 *Zs = reinterpret_cast<C2 *>(smem);
     T *Fy = reinterpret_cast<T *>(smem + align16((size_t)16 * LS * sizeof(C2)));
+    T *At = Fy + KBy * 64;   // GEN: 2 Kp circulant values, then Kp window values
+    T *Wz = At + 2 * Kp;
     // block -> (x run, unit = (y block, z segment)).  Blocks are dealt to the
     // 8 XCDs round robin; the units of one run take consecutive slots of one
     // XCD, so the halo re-reads of neighbouring units meet in its L2.
@@ -742,11 +752,22 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
     {
         auto opy = glb(e.opy) + e.sy.opoff[sy] + (long)yb * KBy * 64;
         for (int i = tid; i < KBy * 64; i += nthr) Fy[i] = opy[i];
+        if (GEN) {
+            auto at = glb(e.zat), w = glb(e.zwin) + (long)sz * a.L;
+            for (int i = tid; i < 2 * a.L; i += nthr) At[i] = at[i];
+            for (int i = tid; i < Kp; i += nthr) Wz[i] = (i < a.L) ? w[i] : T(0);
+        }
     }
     auto opz = glb(a.op) + a.seg.opoff[sz] + (long)(active ? zb : 0) * KBz * 64 + lane;
-    T fz[KZM];
+    T fz[GEN ? 1 : KZM];
+    if (!GEN) {
 #pragma unroll
-    for (int kb = 0; kb < KZM; ++kb) fz[kb] = (KBZ || kb < KBz) ? opz[kb * 64] : T(0);
+        for (int kb = 0; kb < KZM; ++kb) fz[kb] = (KBZ || kb < KBz) ? opz[kb * 64] : T(0);
+    }
+    // GEN: this lane supplies A_z[m][4 kb + kq] with m = zb 16 + cq (zero
+    // past the core); at index i1 + m - l + L stays within 0 .. 2L - 1
+    const bool mz = active && zb * 16 + cq < zn;
+    const int zbase = (zcore - zfft) + (mz ? zb * 16 + cq : 0) - kq + a.L;
     const int Ny = a.Ny, Nz = a.Nz;
     const long plane = (long)Ny * Nz;
     auto psi = glb(reinterpret_cast<const C2 *>(a.psi));
@@ -806,13 +827,25 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
             }
             return v;
         };
+        // the step's operator value: A_y from LDS for a y step; for a z step
+        // (GEN) A_z formed from its factors
+        auto opval = [&](int st) {
+            T f = T(0);
+            if (st < KYM) {
+                if (KBY || st < KBy) f = Fy[st * 64 + lane];
+            } else if (GEN) {
+                const int kb = st - KYM;
+                if (mz && (KBZ || 4 * kb + kq < a.L)) f = At[zbase - 4 * kb] * Wz[4 * kb + kq];
+            }
+            return f;
+        };
         constexpr int NS = KYM + KZM;
         C2 ring[D];
         T fring[D];
 #pragma unroll
         for (int st = 0; st < D; ++st) {
             ring[st] = load(st);
-            fring[st] = (st < KYM && (KBY || st < KBy)) ? Fy[st * 64 + lane] : T(0);
+            fring[st] = opval(st);
         }
 #pragma unroll
         for (int st = 0; st < NS; ++st) {
@@ -820,7 +853,7 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
             const T f = fring[st % D];
             if (st + D < NS) {
                 ring[st % D] = load(st + D);
-                if (st + D < KYM && (KBY || st + D < KBy)) fring[st % D] = Fy[(st + D) * 64 + lane];
+                fring[st % D] = opval(st + D);
             }
             if (st < KYM) {
                 if (KBY || st < KBy) {
@@ -828,8 +861,9 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
                     ai = mfma16(f, v.y, ai);
                 }
             } else if (KBZ || st - KYM < KBz) {
-                ar = mfma16(v.x, fz[st - KYM], ar);
-                ai = mfma16(v.y, fz[st - KYM], ai);
+                const T fzk = GEN ? f : fz[GEN ? 0 : st - KYM];
+                ar = mfma16(v.x, fzk, ar);
+                ai = mfma16(v.y, fzk, ai);
             }
             __builtin_amdgcn_sched_barrier(0);
         }

@@ -67,6 +67,7 @@ struct pstd_solver {
     OpLayout lay[3];
     SegTable seg[3];
     void *op[3] = {nullptr, nullptr, nullptr};
+    void *zat = nullptr, *zwin = nullptr;   // FP64: z operator factors for the fused y + z pass
     void *gvec[3] = {nullptr, nullptr, nullptr};
     double *tfP[3] = {nullptr, nullptr, nullptr};
     double2 *tfA[3] = {nullptr, nullptr, nullptr};
@@ -85,7 +86,7 @@ struct pstd_solver {
     long nTau = 1, nSur = 0;
     int file_prec = PSTD_FP64;    // surface-file precision (header reals and body)
     int nt_mode = -1;             // psi0 streaming: -1 automatic, 0 off, 1 on
-    int yz_mode = -1;             // fused y + z pass: -1 automatic (FP32), 0 never, 1 always
+    int yz_mode = -1;             // fused y + z pass: -1 / 1 wherever it applies, 0 never
     int yz_xrun = 0;              // x planes per wave of the fused pass (0: automatic)
     std::vector<void *> allocs;
     bool timing = false;
@@ -202,6 +203,12 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
             std::vector<long double> frag;
             build_laplacian_fragments(g.ax[a], alpha, s->lay[a], frag);
             if ((rc = upload_real_ld(s, s->op[a], frag, "operator"))) { delete s; return rc; }
+            if (a == 2 && s->esz == 8) {
+                std::vector<long double> at, win;
+                build_laplacian_tables(g.ax[a], alpha, at, win);
+                if ((rc = upload_real_ld(s, s->zat, at, "operator table"))) { delete s; return rc; }
+                if ((rc = upload_real_ld(s, s->zwin, win, "operator window"))) { delete s; return rc; }
+            }
         }
         if (s->fft_ok[a]) {
             if ((rc = upload_real_ld(s, s->fft_tw[a], s->fft[a].tw, "fft twiddles"))) { delete s; return rc; }
@@ -424,24 +431,21 @@ int launch_pass(pstd_solver *s, int axis, const T *psi, T *psi0, long ntau) {
     return 0;
 }
 
-// The y and z passes run as one kernel when both use the register GEMM and
-// the mode asks for it.  Automatically in FP32 only: there the fused kernel
-// is faster (224^3: 0.193 against 0.210 ms per step; 320^3: 0.607 against
-// 0.733), in FP64 it measures the same as the separate passes (0.346
-// against 0.345 ms at 224^3), which then keep FP64 runs identical to the
-// batched passes.
+// The y and z passes run as one kernel when both use the register GEMM,
+// unless the mode says never.  Measured per step at 224^3: FP64 0.313
+// against 0.345 ms, FP32 0.192 against 0.210 ms; 320^3 FP32 0.607 against
+// 0.733 ms.
 bool fuse_yz(const pstd_solver *s) {
     if (s->yz_mode == 0 || s->method[1] != PSTD_METHOD_GEMM || s->method[2] != PSTD_METHOD_GEMM) return false;
-    if (!yz_fusable(s->lay[1].MB, s->lay[1].KB, s->lay[2].MB, s->lay[2].KB)) return false;
-    return s->yz_mode == 1 || s->esz == 4;
+    return yz_fusable(s->lay[1].MB, s->lay[1].KB, s->lay[2].MB, s->lay[2].KB);
 }
 
-// x planes per block of the fused pass (4 by default: several rounds of
-// blocks per CU, so the last round's tail is short, with the operator
-// fragments loaded once per four planes).
+// x planes per block of the fused pass: several rounds of blocks per CU,
+// so the last round's tail is short, with the operator fragments loaded
+// once per run (best measured: 2 FP64, 4 FP32).
 int yz_xrun(const pstd_solver *s) {
     if (s->yz_xrun > 0) return s->yz_xrun;
-    return 4;
+    return s->esz == 8 ? 2 : 4;
 }
 
 template <typename T>
@@ -450,6 +454,8 @@ int launch_fused_yz(pstd_solver *s, const T *psi, T *psi0, long ntau) {
     YZArgs<T> e;
     std::memset(&e, 0, sizeof(e));
     e.opy = reinterpret_cast<const T *>(s->op[1]);
+    e.zat = reinterpret_cast<const T *>(s->zat);
+    e.zwin = reinterpret_cast<const T *>(s->zwin);
     e.sy = s->seg[1];
     e.Nx = (int)s->g.ax[0].N;
     e.Ly = (int)s->g.ax[1].L;

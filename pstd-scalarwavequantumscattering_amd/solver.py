@@ -281,9 +281,8 @@ class PSTDQScat:
         _check(library().pstd_set_cache_streaming(self.h, int(mode)))
 
     def set_pass_fusion(self, mode, xrun=0):
-        """y and z passes as one kernel: 1 wherever both run the GEMM, 0
-        never, -1 (the default) in FP32 only; xrun = x planes per block
-        (0: automatic)."""
+        """y and z passes as one kernel: -1 (the default) or 1 wherever both
+        run the GEMM, 0 never; xrun = x planes per block (0: automatic)."""
         _check(library().pstd_set_pass_fusion(self.h, int(mode), int(xrun)))
 
     def pass_fusion(self):
