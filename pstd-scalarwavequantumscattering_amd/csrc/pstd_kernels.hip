@@ -705,7 +705,7 @@ hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s) {
 // window, staged in LDS): two LDS reads and a multiply per k step.
 template <typename T> struct YZDepth;
 template <> struct YZDepth<double> {
-    static constexpr int D = 4, EQ = 2;
+    static constexpr int D = 4, EQ = 4;
 };
 template <> struct YZDepth<float> {
     static constexpr int D = 8, EQ = 4;
@@ -882,7 +882,7 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
         // arithmetic
 #pragma unroll
         for (int q0 = 0; q0 < 4; q0 += EQ) {
-            C2 acc[4], pn[4];
+            C2 acc[4];
             T vv[4];
 #pragma unroll
             for (int q = q0; q < q0 + EQ; ++q) {
@@ -891,11 +891,8 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
                 const long pt = (long)y * Nz + zce;
                 acc[q] = ld_psi0<NT>(psi0 + (long)x * plane + pt);
                 vv[q] = T(0);
-                pn[q] = czero<C2>();
-                if (vx && y >= a.vb1[1] && y <= a.vb2[1]) {
+                if (vx && y >= a.vb1[1] && y <= a.vb2[1])
                     vv[q] = vbox[((long)(x - a.vb1[0]) * nyv + (y - a.vb1[1])) * nzv + zi.zoff];
-                    pn[q] = ldv(px + pt);
-                }
             }
 #pragma unroll
             for (int q = q0; q < q0 + EQ; ++q) {
@@ -906,8 +903,9 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
                 c.x += ai[q];
                 c.y -= ar[q];
                 const T v = a.vfac * vv[q];
-                c.x += v * pn[q].y;
-                c.y -= v * pn[q].x;
+                const C2 pn = Zs[(r - yb * 16) * LS + (zce - zfft)];   // staged line of y: psi^n at (x, y, z)
+                c.x += v * pn.y;
+                c.y -= v * pn.x;
                 const bool ty = y >= a.k1a[1] && y <= a.k2b[1], ty_in = y > a.k1b[1] && y < a.k2a[1];
                 if ((zi.flags & 2) && tx && ty && !((zi.flags & 4) && tx_in && ty_in)) {
                     // line terms as line_info forms them at unit time factor ...
