@@ -698,17 +698,18 @@ hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s) {
 // read of psi per point and step.  The x term is already in psi0.
 // ---------------------------------------------------------------------------
 // Per wave: D operand loads in flight (a ring, fenced from the scheduler
-// between steps) and EQ rows per epilogue batch.  FP32 keeps its z operator
-// fragments in registers (20).  FP64 would need 40, which caps it at 3 waves
-// per SIMD; it forms each fragment instead from the operator's two factors,
-// A_z[m][l] = at[i1 + m - l + L] W(l) (a circulant row and the segment's
-// window, staged in LDS): two LDS reads and a multiply per k step.
+// between steps); the epilogue's loads go out during the last D steps.
+// FP32 keeps its z operator fragments in registers (20).  FP64 would need
+// 40, which caps it at 3 waves per SIMD; it forms each fragment instead from
+// the operator's two factors, A_z[m][l] = at[i1 + m - l + L] W(l) (a
+// circulant row and the segment's window, staged in LDS): two LDS reads and
+// a multiply per k step.
 template <typename T> struct YZDepth;
 template <> struct YZDepth<double> {
-    static constexpr int D = 4, EQ = 4;
+    static constexpr int D = 4;
 };
 template <> struct YZDepth<float> {
-    static constexpr int D = 8, EQ = 4;
+    static constexpr int D = 8;
 };
 template <typename T> struct YZGen {
     static constexpr bool on = sizeof(T) == 8;
@@ -725,7 +726,7 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
     typedef typename VT<T>::acc ACC;
     constexpr int KYM = KbShape<KBY>::KBM, KZM = KbShape<KBZ>::KBM;
     constexpr int NLD = (16 * 4 * KZM + 255) / 256;   // staging loads per thread (>= 256 threads)
-    constexpr int D = YZDepth<T>::D, EQ = YZDepth<T>::EQ;
+    constexpr int D = YZDepth<T>::D;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const int KBy = KBY ? KBY : e.KBy, KBz = KBZ ? KBZ : a.KB;
     const int Kp = 4 * KBz, LS = Kp + 1;   // staged line stride, odd in complex units
@@ -813,6 +814,30 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
         if (!active) continue;
         const auto px = psi + (long)x * plane;
         ACC ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+        // the lane's row group and column, opaque to loop-invariant code
+        // motion like the operand offset
+        int kqe = kq, zce = zc;
+        asm volatile("" : "+v"(kqe), "+v"(zce));
+        const bool vx = a.use_v && x >= a.vb1[0] && x <= a.vb2[0] && (zi.flags & 1);
+        // the epilogue's loads (psi0 of the four rows, rows past the core
+        // reading the core's last row, and V~), issued during the sweep's last
+        // D steps, when the operand ring issues no more loads
+        C2 acc[4];
+        T vv[4];
+        auto epi_load = [&]() {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = min(yb * 16 + VT<T>::row(kqe, q), yn - 1);
+                const int y = ycore + r;
+                acc[q] = czero<C2>();
+                vv[q] = T(0);
+                if (lcol) {
+                    acc[q] = ld_psi0<NT>(psi0 + (long)x * plane + (long)y * Nz + zce);
+                    if (vx && y >= a.vb1[1] && y <= a.vb2[1])
+                        vv[q] = vbox[((long)(x - a.vb1[0]) * nyv + (y - a.vb1[1])) * nzv + zi.zoff];
+                }
+            }
+        };
         // one software-pipelined sweep: KYM y steps (operand from L2, A_y
         // from LDS), then KZM z steps (operand from LDS, A_z in registers)
         int oy = offy;   // opaque to loop-invariant code motion: hoisted out
@@ -865,37 +890,17 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
                 ar = mfma16(v.x, fzk, ar);
                 ai = mfma16(v.y, fzk, ai);
             }
+            if (st == NS - D) epi_load();
             __builtin_amdgcn_sched_barrier(0);
         }
         // epilogue of the z pass (z_point), with the per-line terms formed
         // here from (x, y) instead of read from the line table
         if (!lcol) continue;
-        // the lane's row group and column, opaque to loop-invariant code
-        // motion like the operand offset
-        int kqe = kq, zce = zc;
-        asm volatile("" : "+v"(kqe), "+v"(zce));
         const double gx = glb(a.g[0])[x];
-        const bool vx = a.use_v && x >= a.vb1[0] && x <= a.vb2[0] && (zi.flags & 1);
         const bool tx = x >= a.k1a[0] && x <= a.k2b[0], tx_in = x > a.k1b[0] && x < a.k2a[0];
-        // the rows in batches of EQ: all loads of a batch first (rows past
-        // the core read the core's last row and are not stored), then the
-        // arithmetic
+        {
 #pragma unroll
-        for (int q0 = 0; q0 < 4; q0 += EQ) {
-            C2 acc[4];
-            T vv[4];
-#pragma unroll
-            for (int q = q0; q < q0 + EQ; ++q) {
-                const int r = min(yb * 16 + VT<T>::row(kqe, q), yn - 1);
-                const int y = ycore + r;
-                const long pt = (long)y * Nz + zce;
-                acc[q] = ld_psi0<NT>(psi0 + (long)x * plane + pt);
-                vv[q] = T(0);
-                if (vx && y >= a.vb1[1] && y <= a.vb2[1])
-                    vv[q] = vbox[((long)(x - a.vb1[0]) * nyv + (y - a.vb1[1])) * nzv + zi.zoff];
-            }
-#pragma unroll
-            for (int q = q0; q < q0 + EQ; ++q) {
+            for (int q = 0; q < 4; ++q) {
                 const int r = yb * 16 + VT<T>::row(kqe, q);
                 if (r >= yn) continue;
                 const int y = ycore + r;
