@@ -228,8 +228,8 @@ int pstd_run_to_steady_state(pstd_solver *s, double tol, long max_periods, long 
  * device runs window k+1 while the host extrapolates from window k, so a
  * run that stops before max_periods has stepped one window further than the
  * windows it used (time index included).  Host memory: rank + 2 copies of the
- * accumulators.  rank 1..16. */
-#define PSTD_RRE_MAX_RANK 16
+ * accumulators.  rank 1..32. */
+#define PSTD_RRE_MAX_RANK 32
 int pstd_run_to_steady_state_rre(pstd_solver *s, double tol, long max_periods, int rank, long *periods_out,
                                  double *history, double *raw_history, int *converged_out);
 

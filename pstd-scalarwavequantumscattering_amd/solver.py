@@ -160,7 +160,7 @@ def compute_geometry(p):
 
 def rre_weights(x):
     """Reduced-rank extrapolation weights of the rows of x (k x n complex,
-    k = 3 .. 18): k - 1 complex weights summing to 1 (pstd_rre_weights)."""
+    k = 3 .. 34): k - 1 complex weights summing to 1 (pstd_rre_weights)."""
     a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
     k, n = a.shape
     out = np.zeros(2 * (k - 1))
@@ -318,7 +318,7 @@ class PSTDQScat:
         n = periods.value
         return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy()
 
-    def run_to_steady_state_rre(self, tol=1e-6, max_periods=100, rank=12):
+    def run_to_steady_state_rre(self, tol=1e-6, max_periods=100, rank=32):
         """Back-to-back windows with reduced-rank extrapolation of the window
         sequence (pstd_run_to_steady_state_rre): stops when successive
         estimates differ by <= tol.  Returns (converged, periods, steps,
@@ -331,7 +331,7 @@ class PSTDQScat:
         n = periods.value
         return bool(conv.value), n, n * self.IterationsToAccumulateSurfaceTerms(), hist[:n].copy(), raw[:n].copy()
 
-    def run_to_steady_state_mixed(self, fp32, switch_tol=1e-4, switch_max=60, tol=1e-6, max_periods=100, rank=12):
+    def run_to_steady_state_mixed(self, fp32, switch_tol=1e-4, switch_max=60, tol=1e-6, max_periods=100, rank=32):
         """This FP64 solver's steady state, started by the FP32 solver fp32 of
         the same problem (pstd_run_to_steady_state_mixed): fp32 runs
         extrapolated windows to switch_tol, hands its state over, and this
