@@ -719,9 +719,10 @@ template <typename T> __host__ __device__ size_t yz_lds_bytes(int KBy, int KBz) 
            (YZGen<T>::on ? (size_t)3 * 4 * KBz * sizeof(T) : 0);
 }
 // KBY, KBZ: compile-time k blocks, for L = 4 KB exactly (no k guards); 0:
-// any KB <= KBMAX read at run time.
+// any KB <= KBMAX read at run time.  The block serves y block yb of y
+// segment sy against z segment sz for x run `run`.
 template <typename T, int KBY, int KBZ, bool NT>
-__global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
+__device__ __forceinline__ void yz_body(const LineArgs<T> &a, const YZArgs<T> &e, int sy, int yb, int sz, int run) {
     typedef typename VT<T>::c2 C2;
     typedef typename VT<T>::acc ACC;
     constexpr int KYM = KbShape<KBY>::KBM, KZM = KbShape<KBZ>::KBM;
@@ -735,16 +736,6 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
     T *Fy = reinterpret_cast<T *>(smem + align16((size_t)16 * LS * sizeof(C2)));
     T *At = Fy + KBy * 64;   // GEN: 2 Kp circulant values, then Kp window values
     T *Wz = At + 2 * Kp;
-    // block -> (x run, unit = (y block, z segment)).  Blocks are dealt to the
-    // 8 XCDs round robin; the units of one run take consecutive slots of one
-    // XCD, so the halo re-reads of neighbouring units meet in its L2.
-    const int b = blockIdx.x, j = b >> 3;
-    const int unit = j % e.nunits, run = (j / e.nunits) * 8 + (b & 7);
-    if (run >= e.nruns) return;   // the whole block leaves
-    int yb = unit / a.seg.n;
-    const int sz = unit - yb * a.seg.n;
-    int sy = 0;
-    while (yb >= e.sy.mb[sy]) yb -= e.sy.mb[sy++];
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, zb = tid >> 6;
     const int kq = lane >> 4, cq = lane & 15;
     const int yfft = e.sy.fft1[sy], ycore = e.sy.core1[sy], yn = e.sy.ncore[sy];
@@ -939,6 +930,78 @@ __global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
             }
         }
     }
+}
+
+template <typename T, int KBY, int KBZ, bool NT>
+__global__ void __launch_bounds__(512) k_lap_yz(LineArgs<T> a, YZArgs<T> e) {
+    // block -> (x run, unit = (y block, z segment)).  Blocks are dealt to the
+    // 8 XCDs round robin; the units of one run take consecutive slots of one
+    // XCD, so the halo re-reads of neighbouring units meet in its L2.
+    const int b = blockIdx.x, j = b >> 3;
+    const int unit = j % e.nunits, run = (j / e.nunits) * 8 + (b & 7);
+    if (run >= e.nruns) return;   // the whole block leaves
+    int yb = unit / a.seg.n;
+    const int sz = unit - yb * a.seg.n;
+    int sy = 0;
+    while (yb >= e.sy.mb[sy]) yb -= e.sy.mb[sy++];
+    yz_body<T, KBY, KBZ, NT>(a, e, sy, yb, sz, run);
+}
+
+// Batched fused y + z pass: blockIdx.y picks a (problem, y segment, y block,
+// z segment) entry of the map, blockIdx.x the x run; the problem's fields are
+// copied into local argument blocks with the two segments' entries at index
+// 0, and the step's time factors come from the batch clock table.
+template <typename T, int KBY, int KBZ, bool NT>
+__global__ void __launch_bounds__(512) k_batch_yz(const LineArgs<T> *__restrict__ list,
+                                                  const YZArgs<T> *__restrict__ elist, const int4 *__restrict__ map,
+                                                  const double *__restrict__ clock, const int *__restrict__ step,
+                                                  int nprob) {
+    const int4 m = map[blockIdx.y];
+    const YZArgs<T> &ge = elist[m.x];
+    const int run = blockIdx.x;
+    if (run >= ge.nruns) return;   // a problem with fewer x runs than the grid
+    const LineArgs<T> &g = list[m.x];
+    const double *c = clock + ((long)(*step) * nprob + m.x) * 3;
+    LineArgs<T> a;
+    a.psi = g.psi;
+    a.psi0 = g.psi0;
+    a.op = g.op;
+    a.nlines = g.nlines;
+    a.Ny = g.Ny;
+    a.Nz = g.Nz;
+    a.L = g.L;
+    a.KB = g.KB;
+    seg_slice<T>(a.seg, g.seg, m.w);
+    a.vbox = g.vbox;
+    a.use_v = g.use_v;
+    a.vfac = (T)c[0];
+    a.sin_t = c[1];
+    a.cos_t = c[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.vb1[d] = g.vb1[d];
+        a.vb2[d] = g.vb2[d];
+        a.tfP[d] = g.tfP[d];
+        a.tfA[d] = g.tfA[d];
+        a.tfE[d] = g.tfE[d];
+        a.k1a[d] = g.k1a[d];
+        a.k1b[d] = g.k1b[d];
+        a.k2a[d] = g.k2a[d];
+        a.k2b[d] = g.k2b[d];
+        a.g[d] = g.g[d];
+    }
+    YZArgs<T> e;
+    e.opy = ge.opy;
+    e.zat = ge.zat;
+    // the z window of segment m.w, at index 0 like the sliced segment tables
+    e.zwin = ge.zwin ? ge.zwin + (long)m.w * g.L : nullptr;
+    seg_slice<T>(e.sy, ge.sy, m.y);
+    e.Nx = ge.Nx;
+    e.Ly = ge.Ly;
+    e.KBy = ge.KBy;
+    e.xrun = ge.xrun;
+    e.nruns = ge.nruns;
+    yz_body<T, KBY, KBZ, NT>(a, e, 0, m.z, 0, run);
 }
 
 // ---------------------------------------------------------------------------
@@ -1395,8 +1458,24 @@ template <typename T> const void *yz_kernel(int KBy, int Ly, int KBz, int Lz, bo
     default: return nt ? (const void *)k_lap_yz<T, 0, 0, true> : (const void *)k_lap_yz<T, 0, 0, false>;
     }
 }
+template <typename T> const void *batch_yz_kernel(int KBy, int Ly, int KBz, int Lz, bool nt) {
+    switch ((KBy == KBz && Ly == 4 * KBy && Lz == 4 * KBz) ? KBy : 0) {
+    case 16: return nt ? (const void *)k_batch_yz<T, 16, 16, true> : (const void *)k_batch_yz<T, 16, 16, false>;
+    case 20: return nt ? (const void *)k_batch_yz<T, 20, 20, true> : (const void *)k_batch_yz<T, 20, 20, false>;
+    case 24: return nt ? (const void *)k_batch_yz<T, 24, 24, true> : (const void *)k_batch_yz<T, 24, 24, false>;
+    default: return nt ? (const void *)k_batch_yz<T, 0, 0, true> : (const void *)k_batch_yz<T, 0, 0, false>;
+    }
+}
 bool yz_fusable(int MBy, int KBy, int MBz, int KBz) {
     return reg_variant_ok(MBy, KBy) && reg_variant_ok(MBz, KBz);
+}
+template <typename T>
+hipError_t launch_batch_yz(const LineArgs<T> *list, const YZArgs<T> *elist, const int4 *map, int nmap,
+                           const double *clock, const int *step, int nprob, int grid_x, int waves, int KBy, int Ly,
+                           int KBz, int Lz, bool uniform, bool nt, hipStream_t s) {
+    void *args[] = {&list, &elist, &map, &clock, &step, &nprob};
+    const void *k = uniform ? batch_yz_kernel<T>(KBy, Ly, KBz, Lz, nt) : batch_yz_kernel<T>(0, 0, 0, 0, nt);
+    return hipLaunchKernel(k, dim3(grid_x, nmap), dim3(64 * waves), args, yz_lds_bytes<T>(KBy, KBz), s);
 }
 template <typename T>
 hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s) {
@@ -1575,6 +1654,8 @@ hipError_t configure_kernels(int max_lds) {
         for (bool nt : {false, true}) {
             k.push_back(yz_kernel<double>(kb, 4 * kb, kb, 4 * kb, nt));
             k.push_back(yz_kernel<float>(kb, 4 * kb, kb, 4 * kb, nt));
+            k.push_back(batch_yz_kernel<double>(kb, 4 * kb, kb, 4 * kb, nt));
+            k.push_back(batch_yz_kernel<float>(kb, 4 * kb, kb, 4 * kb, nt));
         }
     for (const void *f : k) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
@@ -1590,6 +1671,12 @@ template hipError_t launch_plane<float>(const PlaneArgs<float> &, int, hipStream
 template hipError_t launch_line<double>(const LineArgs<double> &, int, hipStream_t);
 template hipError_t launch_line<float>(const LineArgs<float> &, int, hipStream_t);
 template hipError_t launch_yz<double>(const LineArgs<double> &, const YZArgs<double> &, hipStream_t);
+template hipError_t launch_batch_yz<double>(const LineArgs<double> *, const YZArgs<double> *, const int4 *, int,
+                                            const double *, const int *, int, int, int, int, int, int, int, bool,
+                                            bool, hipStream_t);
+template hipError_t launch_batch_yz<float>(const LineArgs<float> *, const YZArgs<float> *, const int4 *, int,
+                                           const double *, const int *, int, int, int, int, int, int, int, bool,
+                                           bool, hipStream_t);
 template hipError_t launch_yz<float>(const LineArgs<float> &, const YZArgs<float> &, hipStream_t);
 template hipError_t launch_surface<double>(const double *, const SurfArgs &, long, hipStream_t);
 template hipError_t launch_surface<float>(const float *, const SurfArgs &, long, hipStream_t);

@@ -121,6 +121,14 @@ template <typename T> hipError_t launch_line(const LineArgs<T> &a, int grid_x, h
 template <typename T> hipError_t launch_surface(const T *psi, const SurfArgs &a, long max_pts, hipStream_t s);
 // fused y + z pass; needs the register GEMM variant on both axes (yz_fusable)
 template <typename T> hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s);
+// batched fused pass: map entries (problem, y segment, y block, z segment),
+// grid_x x runs (the most of any problem), waves per block >= every z
+// segment's 16-row blocks; KB / L the batch's largest (uniform: every
+// problem has them, so the compile-time instance may be used)
+template <typename T>
+hipError_t launch_batch_yz(const LineArgs<T> *list, const YZArgs<T> *elist, const int4 *map, int nmap,
+                           const double *clock, const int *step, int nprob, int grid_x, int waves, int KBy, int Ly,
+                           int KBz, int Lz, bool uniform, bool nt, hipStream_t s);
 bool yz_fusable(int MBy, int KBy, int MBz, int KBz);
 template <typename T> size_t pass_lds_bytes(int MB, int KB);
 template <typename T> int pass_occupancy(int MB, int KB, bool line);

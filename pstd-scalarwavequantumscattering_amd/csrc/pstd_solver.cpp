@@ -449,8 +449,7 @@ int yz_xrun(const pstd_solver *s) {
 }
 
 template <typename T>
-int launch_fused_yz(pstd_solver *s, const T *psi, T *psi0, long ntau) {
-    LineArgs<T> a = line_args<T>(s, psi, psi0, ntau);
+YZArgs<T> yz_args(const pstd_solver *s) {
     YZArgs<T> e;
     std::memset(&e, 0, sizeof(e));
     e.opy = reinterpret_cast<const T *>(s->op[1]);
@@ -467,6 +466,13 @@ int launch_fused_yz(pstd_solver *s, const T *psi, T *psi0, long ntau) {
     e.zw = zmax;
     e.xrun = std::min(e.Nx, yz_xrun(s));
     e.nruns = (e.Nx + e.xrun - 1) / e.xrun;
+    return e;
+}
+
+template <typename T>
+int launch_fused_yz(pstd_solver *s, const T *psi, T *psi0, long ntau) {
+    LineArgs<T> a = line_args<T>(s, psi, psi0, ntau);
+    YZArgs<T> e = yz_args<T>(s);
     TMARK(s, 2);
     HIPCHK(launch_yz<T>(a, e, s->stream), "fused y + z pass");
     TMARK(s, 2);
@@ -1382,6 +1388,13 @@ struct pstd_batch {
     void *lists[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [parity][pass]
     int2 *maps[3] = {nullptr, nullptr, nullptr};
     int nmap[3] = {0, 0, 0};
+    // fused y + z pass over all members (when every member fuses): per-problem
+    // y operator arguments, the (problem, y segment, y block, z segment) map,
+    // x runs of the largest problem, waves per block
+    bool fused = false, yz_uniform = false;
+    void *ylist = nullptr;
+    int4 *ymap = nullptr;
+    int nymap = 0, ygx = 1, ywaves = 4;
     int gx[3] = {8, 8, 8};
     int tiles_per_block = 16;
     double *clock = nullptr;      // [step][problem][vfac, sin t, cos t]
@@ -1401,6 +1414,8 @@ struct pstd_batch {
                 if (lists[p][k]) (void)hipFree(lists[p][k]);
         for (int k = 0; k < 3; ++k)
             if (maps[k]) (void)hipFree(maps[k]);
+        if (ylist) (void)hipFree(ylist);
+        if (ymap) (void)hipFree(ymap);
         if (clock) (void)hipFree(clock);
         if (step) (void)hipFree(step);
         for (pstd_solver *s : sv) delete s;
@@ -1468,6 +1483,31 @@ int batch_build(pstd_batch *b) {
         HIPCHK(hipMemcpy(b->lists[p][1], ly.data(), ly.size() * sizeof(PlaneArgs<T>), hipMemcpyHostToDevice), "batch args");
         HIPCHK(hipMemcpy(b->lists[p][2], lz.data(), lz.size() * sizeof(LineArgs<T>), hipMemcpyHostToDevice), "batch args");
     }
+    // the fused y + z pass, when every member runs it
+    b->fused = true;
+    for (pstd_solver *s : b->sv) b->fused = b->fused && fuse_yz(s);
+    if (b->fused) {
+        std::vector<YZArgs<T>> el;
+        std::vector<int4> ym;
+        b->yz_uniform = true;
+        const int KB0 = b->sv[0]->lay[1].KB;
+        for (int i = 0; i < nprob; ++i) {
+            pstd_solver *s = b->sv[i];
+            el.push_back(yz_args<T>(s));
+            b->ygx = std::max(b->ygx, el.back().nruns);
+            b->ywaves = std::max(b->ywaves, el.back().zw);
+            for (int sy = 0; sy < s->seg[1].n; ++sy)
+                for (int yb = 0; yb < s->seg[1].mb[sy]; ++yb)
+                    for (int sz = 0; sz < s->seg[2].n; ++sz) ym.push_back(make_int4(i, sy, yb, sz));
+            b->yz_uniform = b->yz_uniform && s->lay[1].KB == KB0 && s->lay[2].KB == KB0 &&
+                            s->g.ax[1].L == 4 * KB0 && s->g.ax[2].L == 4 * KB0;
+        }
+        b->nymap = (int)ym.size();
+        HIPCHK(hipMalloc(&b->ylist, el.size() * sizeof(YZArgs<T>)), "batch args");
+        HIPCHK(hipMemcpy(b->ylist, el.data(), el.size() * sizeof(YZArgs<T>), hipMemcpyHostToDevice), "batch args");
+        HIPCHK(hipMalloc(&b->ymap, ym.size() * sizeof(int4)), "batch map");
+        HIPCHK(hipMemcpy(b->ymap, ym.data(), ym.size() * sizeof(int4), hipMemcpyHostToDevice), "batch map");
+    }
     for (int k = 0; k < 3; ++k) {
         b->nmap[k] = (int)maps[k].size();
         HIPCHK(hipMalloc(&b->maps[k], maps[k].size() * sizeof(int2)), "batch map");
@@ -1493,6 +1533,15 @@ int batch_step_launch(pstd_batch *b, int parity) {
                                  b->gx[0], b->block, KB, b->kb_uniform[0] && b->sv[0]->lay[0].KB == KB, b->nt,
                                  b->stream),
            "batch x pass");
+    if (b->fused) {
+        HIPCHK(launch_batch_yz<T>(reinterpret_cast<const LineArgs<T> *>(b->lists[parity][2]),
+                                  reinterpret_cast<const YZArgs<T> *>(b->ylist), b->ymap, b->nymap, b->clock, b->step,
+                                  nprob, b->ygx, b->ywaves, KB, 4 * KB, KB, 4 * KB, b->yz_uniform && b->sv[0]->lay[1].KB == KB,
+                                  b->nt, b->stream),
+               "batch fused y + z pass");
+        HIPCHK(launch_tick(b->step, b->stream), "batch tick");
+        return 0;
+    }
     HIPCHK(launch_batch_plane<T>(reinterpret_cast<const PlaneArgs<T> *>(b->lists[parity][1]), b->maps[1], b->nmap[1],
                                  b->gx[1], b->block, KB, b->kb_uniform[1] && b->sv[0]->lay[1].KB == KB, b->nt,
                                  b->stream),

@@ -25,8 +25,7 @@ def test_batch_matches_solo_runs(precision):
     print(f"case batch of {len(probs)} prec={precision}", flush=True)
     batch = pstd_amd.PSTDBatch([p for p, _ in probs], [v for _, v in probs])
     solo = [pstd_amd.PSTDQScat(p, v) for p, v in probs]
-    for s in solo:
-        s.set_pass_fusion(0)      # the batch runs the y and z passes separately
+    assert all(s.pass_fusion() == 1 for s in solo)   # the batch fuses the y and z passes as they do
     tol = 1e-13 if precision == pstd_amd.PSTD_FP64 else 1e-6
     # graph replays (2 steps each) with eager host work between them, then an
     # odd remainder stepped eagerly
@@ -69,8 +68,6 @@ def test_streamed_batch_matches_solo_runs():
     for i, m in enumerate(batch.members):
         pick.setdefault(m.N[0], i)                  # one member per grid size
     solo = {i: pstd_amd.PSTDQScat(*probs[i]) for i in pick.values()}
-    for s in solo.values():
-        s.set_pass_fusion(0)
     batch.Update(5)
     batch.update_accumulate(2)
     for i, s in solo.items():
