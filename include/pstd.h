@@ -244,8 +244,8 @@ int pstd_run_to_steady_state_rre(pstd_solver *s, double tol, long max_periods, i
  * raw_history and converged_out as in pstd_run_to_steady_state_rre, for the
  * FP64 phase).  The FP32 phase's error (about 1e-4 on these workloads)
  * leaves with the transient of the FP64 phase: on the 224^3 Gaussian random
- * potential the result matches the FP64-only run's estimate to 6e-8 in
- * about 20 % less time.  periods32_out and periods64_out receive the windows
+ * potential, run to a 1e-7 change, the result is as close to a 400-period
+ * FP64 estimate (1.05e-6) as the FP64-only run's, in 11 % less time.  periods32_out and periods64_out receive the windows
  * of each phase; s64's accumulators hold the estimate. */
 int pstd_run_to_steady_state_mixed(pstd_solver *s32, pstd_solver *s64, double switch_tol, long switch_max,
                                    double tol, long max_periods, int rank, long *periods32_out,
