@@ -164,3 +164,35 @@ def test_create_without_gpu_fails_loudly():
     with pytest.raises(pstd_amd.PSTDError) as e:
         pstd_amd.PSTDQScat(p, V)
     assert e.value.code == pstd_amd.PSTD_ENODEV
+
+
+def test_null_solver_arguments_are_rejected():
+    """Entry points taking solvers reject null handles with 22 before any
+    device work (no GPU needed)."""
+    lib = pstd_amd.library()
+    assert lib.pstd_set_pass_fusion(None, -1, 0) == pstd_amd.PSTD_EINVAL
+    assert lib.pstd_get_pass_fusion(None) == -1
+    assert lib.pstd_set_cache_streaming(None, 0) == pstd_amd.PSTD_EINVAL
+    p32, p64, conv = C.c_long(0), C.c_long(0), C.c_int(0)
+    hist = (C.c_double * 4)()
+    rc = lib.pstd_run_to_steady_state_mixed(None, None, 1e-4, 4, 1e-6, 4, 8, C.byref(p32), C.byref(p64), hist, hist,
+                                            C.byref(conv))
+    assert rc == pstd_amd.PSTD_EINVAL
+
+
+def test_rre_weights_accept_rank_32():
+    """pstd_rre_weights takes up to PSTD_RRE_MAX_RANK + 2 = 34 vectors: on
+    x_j = x* + c r^j (one geometric mode) the weights reproduce x* exactly,
+    and 35 vectors are rejected."""
+    rng = np.random.default_rng(4)
+    xs = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    c = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    d = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    x = np.stack([xs + c * 0.7 ** j + d * (-0.4) ** j for j in range(34)])
+    g = pstd_amd.rre_weights(x)
+    assert len(g) == 33 and abs(g.sum() - 1) < 1e-12
+    est = (g[:, None] * x[:33]).sum(axis=0)
+    assert np.linalg.norm(est - xs) / np.linalg.norm(xs) < 1e-8
+    with pytest.raises(pstd_amd.PSTDError) as e:
+        pstd_amd.rre_weights(np.vstack([x, x[:1]]))
+    assert e.value.code == pstd_amd.PSTD_EINVAL
