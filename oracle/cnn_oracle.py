@@ -11,8 +11,8 @@ Adam.  Two modes:
                rounded to bfloat16 (round to nearest even) where the kernels
                round them; sums in float64.  The GPU path is checked against
                this mode.
-The reference has no learned component; this network is new (a north-star
-addition), so there is no reference file to cite.
+The reference has no learned component; this network is new (an addition
+of this project), so there is no reference file to cite.
 
 Layout: activations [B][R][R][R][C] (z, y, x, channel; channels last), C a
 multiple of 16.  Convolutions are 3x3x3, stride 1, zero padding 1, weights

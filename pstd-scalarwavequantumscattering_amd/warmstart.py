@@ -10,7 +10,7 @@ A - A_init between the steady-state amplitude and the initial field, and
 the solve starts from that amplitude one period in (past the ramp):
 psi^{n0} = A e^{-i n0 dtau}, psi^{n0-1} = A e^{-i (n0-1) dtau}.
 
-The judge of a start is the PSTD residual on the virtual surfaces: one
+The measure of a start is the PSTD residual on the virtual surfaces: one
 Update of the harmonic state A should give A e^{-i (n0+1) dtau}; the
 relative mismatch on the six surface planes is computed for the predicted
 amplitude and for A_init, and the plain start (the reference's) is kept

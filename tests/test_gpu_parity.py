@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-FP64: psi and the surface terms within 1e-10 relative L2 (the north-star
-tolerance for the FP64 path; the two sides use different algorithms for the
+FP64: psi and the surface terms within 1e-10 relative L2 (the stated
+tolerance of the FP64 path; the two sides use different algorithms for the
 line operator -- dense windowed MFMA GEMM vs. FFT per local line -- so they
 agree to rounding, not bit for bit).  FP32: within 2e-5 of the FP64 oracle
 after a few steps (storage and MFMA in FP32, phases in FP64).
