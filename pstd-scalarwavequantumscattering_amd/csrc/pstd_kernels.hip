@@ -714,9 +714,17 @@ template <> struct YZDepth<float> {
 template <typename T> struct YZGen {
     static constexpr bool on = sizeof(T) == 8;
 };
-template <typename T> __host__ __device__ size_t yz_lds_bytes(int KBy, int KBz) {
-    return align16((size_t)16 * (4 * KBz + 1) * 2 * sizeof(T)) + (size_t)KBy * 64 * sizeof(T) +
-           (YZGen<T>::on ? (size_t)3 * 4 * KBz * sizeof(T) : 0);
+// TF/SF factor records of the fused pass (FP64, 6 doubles each): per plane
+// and y row {E_x E_y, A_x P_y + P_x A_y, P_x P_y}, per z column {E_z times
+// the step's time factor, A_z, P_z}; then the ABC factor g1 of the 16 rows
+constexpr int YZ_TREC = 6;
+template <typename T> __host__ __device__ size_t yz_tab_offset(int KBy, int KBz) {
+    return align16(align16((size_t)16 * (4 * KBz + 1) * 2 * sizeof(T)) + (size_t)KBy * 64 * sizeof(T) +
+                   (YZGen<T>::on ? (size_t)3 * 4 * KBz * sizeof(T) : 0));
+}
+// zw = waves per block (16 z columns each)
+template <typename T> __host__ __device__ size_t yz_lds_bytes(int KBy, int KBz, int zw) {
+    return yz_tab_offset<T>(KBy, KBz) + ((size_t)(16 + 16 * zw) * YZ_TREC + 16) * sizeof(double);
 }
 // KBY, KBZ: compile-time k blocks, for L = 4 KB exactly (no k guards); 0:
 // any KB <= KBMAX read at run time.  The block serves y block yb of y
@@ -736,6 +744,9 @@ __device__ __forceinline__ void yz_body(const LineArgs<T> &a, const YZArgs<T> &e
     T *Fy = reinterpret_cast<T *>(smem + align16((size_t)16 * LS * sizeof(C2)));
     T *At = Fy + KBy * 64;   // GEN: 2 Kp circulant values, then Kp window values
     T *Wz = At + 2 * Kp;
+    double *Txy = reinterpret_cast<double *>(smem + yz_tab_offset<T>(KBy, KBz));   // 16 rows
+    double *Tz = Txy + 16 * YZ_TREC;                                                // the block's z columns
+    double *Gy = Tz + (blockDim.x >> 6) * 16 * YZ_TREC;                             // g1 of the 16 rows
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, zb = tid >> 6;
     const int kq = lane >> 4, cq = lane & 15;
     const int yfft = e.sy.fft1[sy], ycore = e.sy.core1[sy], yn = e.sy.ncore[sy];
@@ -748,6 +759,21 @@ __device__ __forceinline__ void yz_body(const LineArgs<T> &a, const YZArgs<T> &e
             auto at = glb(e.zat), w = glb(e.zwin) + (long)sz * a.L;
             for (int i = tid; i < 2 * a.L; i += nthr) At[i] = at[i];
             for (int i = tid; i < Kp; i += nthr) Wz[i] = (i < a.L) ? w[i] : T(0);
+        }
+        if (tid < 16) Gy[tid] = glb(a.g[1])[ycore + min(yb * 16 + tid, yn - 1)];
+        // z factors of the block's columns zcore + j inside the TF box (read
+        // only there), E_z times (sin t + i cos t) as in the separate z pass
+        for (int j = tid; j < (nthr >> 6) * 16; j += nthr) {
+            const int z = zcore + min(j, zn - 1);
+            double *t = Tz + j * YZ_TREC;
+            if (z >= a.k1a[2] && z <= a.k2b[2]) {
+                const ZSrc zs = z_src<T>(a, z);
+                t[0] = zs.ez.x * a.sin_t - zs.ez.y * a.cos_t;
+                t[1] = zs.ez.x * a.cos_t + zs.ez.y * a.sin_t;
+                t[2] = zs.az.x;
+                t[3] = zs.az.y;
+                t[4] = zs.pz;
+            }
         }
     }
     auto opz = glb(a.op) + a.seg.opoff[sz] + (long)(active ? zb : 0) * KBz * 64 + lane;
@@ -791,13 +817,28 @@ __device__ __forceinline__ void yz_body(const LineArgs<T> &a, const YZArgs<T> &e
     const int x1 = run * e.xrun, x2 = min(e.Nx, x1 + e.xrun);
     if (x1 < x2) fetch(x1);
     for (int x = x1; x < x2; ++x) {
-        __syncthreads();   // the previous plane's staged lines are consumed
+        __syncthreads();   // the previous plane's staged lines and factors are consumed
 #pragma unroll
         for (int jj = 0; jj < NLD; ++jj) {
             const int i = tid + jj * nthr;
             if (i < nst) {
                 const int line = i / Kp, l = i - line * Kp;
                 Zs[line * LS + l] = pre[jj];
+            }
+        }
+        if (tid < 16 && x >= a.k1a[0] && x <= a.k2b[0]) {
+            // x y factors of the plane's row tid (the line terms of line_info)
+            const int y = ycore + min(yb * 16 + tid, yn - 1);
+            if (y >= a.k1a[1] && y <= a.k2b[1]) {
+                const double pxv = glb(a.tfP[0])[x], pyv = glb(a.tfP[1])[y];
+                const double2 ax = ldv(glb(a.tfA[0]) + x), ay = ldv(glb(a.tfA[1]) + y);
+                const double2 ex = ldv(glb(a.tfE[0]) + x), ey = ldv(glb(a.tfE[1]) + y);
+                double *t = Txy + tid * YZ_TREC;
+                t[0] = ex.x * ey.x - ex.y * ey.y;
+                t[1] = ex.x * ey.y + ex.y * ey.x;
+                t[2] = ax.x * pyv + ay.x * pxv;
+                t[3] = ax.y * pyv + ay.y * pxv;
+                t[4] = pxv * pyv;
             }
         }
         __syncthreads();
@@ -904,26 +945,20 @@ __device__ __forceinline__ void yz_body(const LineArgs<T> &a, const YZArgs<T> &e
                 c.y -= v * pn.x;
                 const bool ty = y >= a.k1a[1] && y <= a.k2b[1], ty_in = y > a.k1b[1] && y < a.k2a[1];
                 if ((zi.flags & 2) && tx && ty && !((zi.flags & 4) && tx_in && ty_in)) {
-                    // line terms as line_info forms them at unit time factor ...
-                    const double pxv = glb(a.tfP[0])[x], pyv = glb(a.tfP[1])[y];
-                    const double2 ax = ldv(glb(a.tfA[0]) + x), ay = ldv(glb(a.tfA[1]) + y);
-                    const double2 ex = ldv(glb(a.tfE[0]) + x), ey = ldv(glb(a.tfE[1]) + y);
-                    const double pxy = pxv * pyv;
-                    const double cx = ax.x * pyv + ay.x * pxv, cy = ax.y * pyv + ay.y * pxv;
-                    const double exr = ex.x * ey.x - ex.y * ey.y, exi = ex.x * ey.y + ex.y * ey.x;
-                    // ... and the z terms with the time factor (sin t + i cos t)
-                    // folded into E_z, as the register z pass does
-                    const ZSrc zs = z_src<T>(a, zce);
-                    const double ezr = zs.ez.x * a.sin_t - zs.ez.y * a.cos_t;
-                    const double ezi = zs.ez.x * a.cos_t + zs.ez.y * a.sin_t;
-                    const double ur2 = cx * zs.pz + pxy * zs.az.x;
-                    const double ui2 = cy * zs.pz + pxy * zs.az.y;
+                    // line terms as line_info forms them at unit time factor,
+                    // and the z terms with the time factor folded into E_z, as
+                    // the register z pass does; both from the LDS records
+                    const double *txy = Txy + (r - yb * 16) * YZ_TREC, *tz = Tz + (zce - zcore) * YZ_TREC;
+                    const double exr = txy[0], exi = txy[1], cx = txy[2], cy = txy[3], pxy = txy[4];
+                    const double ezr = tz[0], ezi = tz[1];
+                    const double ur2 = cx * tz[4] + pxy * tz[2];
+                    const double ui2 = cy * tz[4] + pxy * tz[3];
                     const double er = exr * ezr - exi * ezi;
                     const double ei = exr * ezi + exi * ezr;
                     c.x -= (T)(er * ur2 - ei * ui2);
                     c.y -= (T)(er * ui2 + ei * ur2);
                 }
-                const double gg = gx * glb(a.g[1])[y] * zi.gz;
+                const double gg = gx * Gy[r - yb * 16] * zi.gz;
                 c.x = (T)(c.x * gg);
                 c.y = (T)(c.y * gg);
                 st_psi0<NT>(psi0 + (long)x * plane + (long)y * Nz + zce, c);
@@ -1475,7 +1510,7 @@ hipError_t launch_batch_yz(const LineArgs<T> *list, const YZArgs<T> *elist, cons
                            int KBz, int Lz, bool uniform, bool nt, hipStream_t s) {
     void *args[] = {&list, &elist, &map, &clock, &step, &nprob};
     const void *k = uniform ? batch_yz_kernel<T>(KBy, Ly, KBz, Lz, nt) : batch_yz_kernel<T>(0, 0, 0, 0, nt);
-    return hipLaunchKernel(k, dim3(grid_x, nmap), dim3(64 * waves), args, yz_lds_bytes<T>(KBy, KBz), s);
+    return hipLaunchKernel(k, dim3(grid_x, nmap), dim3(64 * waves), args, yz_lds_bytes<T>(KBy, KBz, waves), s);
 }
 template <typename T>
 hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s) {
@@ -1484,7 +1519,7 @@ hipError_t launch_yz(const LineArgs<T> &a, const YZArgs<T> &e, hipStream_t s) {
     YZArgs<T> ext = e;
     void *args[] = {&arg, &ext};
     return hipLaunchKernel(yz_kernel<T>(e.KBy, e.Ly, a.KB, a.L, a.nt != 0), dim3((unsigned)blocks), dim3(64 * e.zw),
-                           args, yz_lds_bytes<T>(e.KBy, a.KB), s);
+                           args, yz_lds_bytes<T>(e.KBy, a.KB, e.zw), s);
 }
 
 template <typename T>
