@@ -699,20 +699,22 @@ hipError_t launch_widen(const void *src, void *dst, long n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Per wave: D operand loads in flight (a ring, fenced from the scheduler
 // between steps); the epilogue's loads go out during the last D steps.
-// FP32 keeps its z operator fragments in registers (20).  FP64 would need
-// 40, which caps it at 3 waves per SIMD; it forms each fragment instead from
-// the operator's two factors, A_z[m][l] = at[i1 + m - l + L] W(l) (a
-// circulant row and the segment's window, staged in LDS): two LDS reads and
-// a multiply per k step.
+// Held in registers, the z operator fragments would take 40 registers in
+// FP64 (3 waves per SIMD) and 20 in FP32 (4).  Each fragment is formed
+// instead from the operator's two factors, A_z[m][l] = at[i1 + m - l + L]
+// W(l) (a circulant row and the segment's window, staged in LDS): two LDS
+// reads and a multiply per k step.  FP64 then runs 4 waves per SIMD, FP32
+// with a ring of 6 loads 5 (93 registers).  YZGen::on = false keeps the
+// fragments in registers.
 template <typename T> struct YZDepth;
 template <> struct YZDepth<double> {
     static constexpr int D = 4;
 };
 template <> struct YZDepth<float> {
-    static constexpr int D = 8;
+    static constexpr int D = 6;
 };
 template <typename T> struct YZGen {
-    static constexpr bool on = sizeof(T) == 8;
+    static constexpr bool on = true;
 };
 // TF/SF factor records of the fused pass (FP64, 6 doubles each): per plane
 // and y row {E_x E_y, A_x P_y + P_x A_y, P_x P_y}, per z column {E_z times

@@ -94,7 +94,7 @@ struct LineArgs {
 template <typename T>
 struct YZArgs {
     const T *opy;
-    const T *zat, *zwin;  // FP64: the z operator as factors (build_laplacian_tables)
+    const T *zat, *zwin;  // the z operator as factors (build_laplacian_tables)
     SegTable sy;
     int Nx, Ly, KBy;
     int nunits;           // (16-row y blocks summed over the y segments) x (z segments)

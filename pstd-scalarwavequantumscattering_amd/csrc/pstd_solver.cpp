@@ -67,7 +67,7 @@ struct pstd_solver {
     OpLayout lay[3];
     SegTable seg[3];
     void *op[3] = {nullptr, nullptr, nullptr};
-    void *zat = nullptr, *zwin = nullptr;   // FP64: z operator factors for the fused y + z pass
+    void *zat = nullptr, *zwin = nullptr;   // z operator factors for the fused y + z pass
     void *gvec[3] = {nullptr, nullptr, nullptr};
     double *tfP[3] = {nullptr, nullptr, nullptr};
     double2 *tfA[3] = {nullptr, nullptr, nullptr};
@@ -203,7 +203,7 @@ int create_common(const pstd_params *p, const double *vbox_host, size_t nv, pstd
             std::vector<long double> frag;
             build_laplacian_fragments(g.ax[a], alpha, s->lay[a], frag);
             if ((rc = upload_real_ld(s, s->op[a], frag, "operator"))) { delete s; return rc; }
-            if (a == 2 && s->esz == 8) {
+            if (a == 2) {
                 std::vector<long double> at, win;
                 build_laplacian_tables(g.ax[a], alpha, at, win);
                 if ((rc = upload_real_ld(s, s->zat, at, "operator table"))) { delete s; return rc; }
